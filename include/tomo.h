@@ -1,0 +1,205 @@
+/*
+ * tomo.h — C ABI of the MI355X-native 2D TMz FDTD fitness evaluation.
+ *
+ * The reference exposes exactly one interface on this path:
+ *     double FDTD_GPU(const std::vector<double>&)   (FDTD_GPU.cuh:8, FDTD_GPU.cu:650)
+ *     double FDTD_CPU(const std::vector<double>&)   (FDTD_CPU.h:8,  FDTD_CPU.cxx:669)
+ * a free C++ function that maps a PSO/DE parameter vector to a fitness (higher
+ * is better), allocating and freeing everything inside the call, with all
+ * physics fixed by compile-time macros (FDTD_common.hxx:3-65).
+ *
+ * This header replaces that with plain C entry points (no C++ types, no torch
+ * types, status codes instead of exit(-1)):
+ *   - tomo_sim_config : the run-time replacement of the FDTD_common.hxx macros;
+ *   - tomo_preset     : the reference configuration and the benchmark workloads;
+ *   - tomo_eval       : one fitness evaluation, the C-ABI twin of FDTD_GPU();
+ *   - tomo_eval_batch : many evaluations in one batched launch sequence;
+ *   - tomo_submit/poll/wait/progress : continuous-batching request loop;
+ *   - tomo_sim_*      : single-simulation access used by parity tests
+ *                       (fields, running-DFT accumulators, material map).
+ * The C++ drop-in shim `double FDTD_GPU(const std::vector<double>&)` is
+ * exported by the same library (see tomography_amd/csrc/fdtd_gpu_shim.cpp).
+ *
+ * Every function returns TOMO_OK (0) or a negative TOMO_E* code; the message
+ * of the last failure on the calling thread is available from
+ * tomo_last_error().  Nothing throws across this boundary and nothing exits.
+ */
+#ifndef TOMO_H
+#define TOMO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TOMO_OK 0
+#define TOMO_E_INVALID -1   /* bad argument / configuration / parameter     */
+#define TOMO_E_SHORT -2     /* parameter vector too short (reference: vector::at throws) */
+#define TOMO_E_HIP -3       /* HIP runtime error                            */
+#define TOMO_E_NOMEM -4     /* HBM budget or allocation failure             */
+#define TOMO_E_NONFINITE -5 /* NaN/Inf detected in a physical-mode result   */
+#define TOMO_E_CANCELLED -6 /* request cancelled                           */
+#define TOMO_E_PENDING -7   /* request not finished yet (tomo_poll)         */
+#define TOMO_E_STATE -8     /* call not valid in the current state          */
+
+/* ---- modes and flags --------------------------------------------------- */
+/* Reference-compatible physics: scattered-field plane-wave excitation, the
+ * reference CPML quirks, Debye Jp as in E_field_update (FDTD_GPU.cu:358-434),
+ * NaN clamp, NTFF/far field as in N2FPostProcess (FDTD_GPU.cu:553-634). */
+#define TOMO_MODE_REF_COMPAT 0
+/* Physical mode (benchmark workloads): total field, soft differentiated-
+ * Gaussian point source per antenna, standard CPML, Debye ADE everywhere. */
+#define TOMO_MODE_PHYSICAL 1
+
+/* Reference CPU-path variants (FDTD_CPU.cxx), selectable in ref-compat mode. */
+#define TOMO_FLAG_FWF_ABS_ORIGIN 0x1  /* incident wave uses absolute cell indices (FDTD_CPU.cxx:316) */
+#define TOMO_FLAG_CPU_COEFS 0x2       /* no Debye; Cezeic/Cezeip CPU form (FDTD_CPU.cxx:893-905)      */
+#define TOMO_FLAG_NO_NAN_CLAMP 0x4    /* do not clamp NaN to 0 (FDTD_CPU.cxx:272-277)                 */
+#define TOMO_FLAG_PARAMS_EPS_SIGMA 0x8 /* parameter layout {eps_r[n^2], sigma[n^2]} (FDTD_CPU.cxx:681-698) */
+#define TOMO_FLAG_PHASE_FITNESS 0x10  /* add the phase term sketched at FDTD_GPU.cu:1009-1018          */
+#define TOMO_FLAG_DFT_FP64 0x20       /* running-DFT accumulators in FP64 instead of Kahan FP32      */
+#define TOMO_FLAG_CPU_REF_VARIANT                                                              \
+    (TOMO_FLAG_FWF_ABS_ORIGIN | TOMO_FLAG_CPU_COEFS | TOMO_FLAG_NO_NAN_CLAMP |             \
+     TOMO_FLAG_PARAMS_EPS_SIGMA)
+
+/* ---- presets ----------------------------------------------------------- */
+#define TOMO_PRESET_REF_GPU 0   /* FDTD_common.hxx macros + FDTD_GPU semantics, 27 parameters  */
+#define TOMO_PRESET_REF_CPU 1   /* FDTD_common.hxx macros + FDTD_CPU semantics, 162 parameters */
+#define TOMO_PRESET_CFG1 2      /* 256^2, dx 0.5 mm, 2048 steps, 1 antenna, 8 f x 180 angles   */
+#define TOMO_PRESET_CFG2 3      /* 1024^2, dx 0.25 mm, 8192 steps, 1 antenna                   */
+#define TOMO_PRESET_CFG3 4      /* 512^2, dx 0.5 mm, 4096 steps, 16 antennas, 16 f x 360 angles */
+#define TOMO_PRESET_CFG4 5      /* uneven-batch template: 256^2..2048^2 chosen per request      */
+#define TOMO_PRESET_CFG5 6      /* serving template (same physics as CFG4)                      */
+
+/*
+ * Run-time replacement of the FDTD_common.hxx macros.  Geometry is square in
+ * every preset; nx != ny is accepted in physical mode only.
+ */
+typedef struct tomo_sim_config {
+    int32_t mode;        /* TOMO_MODE_*                                          */
+    int32_t flags;       /* TOMO_FLAG_*                                          */
+    int32_t nx, ny;      /* Yee cells; Ez nodes are (nx+1) x (ny+1)              */
+    int32_t steps;       /* time steps per excitation (number_of_time_steps)     */
+    int32_t n_exc;       /* excitation angles (ref) or antennas (physical)       */
+    int32_t ncpml;       /* CPML thickness in cells (ncells)                     */
+    int32_t ntff_offset; /* NTFF contour distance from the edge (NF2FFdistfromboundary) */
+    int32_t n_freq;      /* numberoffrequencies                                  */
+    int32_t n_angles;    /* numberofobservationangles                            */
+    int32_t numpatches;  /* ref: patch grid edge (3 GPU path, 9 CPU path)        */
+    int32_t n_params;    /* minimum parameter-vector length                      */
+    double dx, dy, dt;
+    double f_start, f_stop;       /* startfrequency, stopfrequency               */
+    double cpml_order;            /* npml (ref) / grading order m (physical)     */
+    double kappa_max;             /* kmax                                        */
+    double alpha_max;             /* alpha_max                                   */
+    double sigma_max;             /* sigma_max (ref: (npml+1)/(150 pi dx))        */
+    double pulse_tau, pulse_t0;   /* tau, t0 of the excitation pulse             */
+    double debye_tau;             /* ref: relaxation_time                        */
+    double breast_radius;         /* ref: disc radius; physical: outer radius    */
+    double skin_thickness;        /* physical                                    */
+    double src_radius;            /* physical: antenna ring radius               */
+    double ff_norm;               /* ref: the 33.329 far-field normaliser        */
+    uint64_t seed;                /* physical: phantom seed                      */
+} tomo_sim_config;
+
+/* Fill *out with a preset.  `grid` overrides nx = ny (and the step count as
+ * clamp(8*grid, 2048, 16384)) for the CFG4/CFG5 templates; pass 0 otherwise. */
+int tomo_preset(int preset, int grid, tomo_sim_config *out);
+
+/* Length of the far-field record of one evaluation: n_exc * n_freq * n_angles,
+ * laid out [exc][freq][angle] exactly like D in FDTD_GPU.cu:975. */
+int64_t tomo_farfield_len(const tomo_sim_config *cfg);
+
+/* Number of NTFF contour points: 2*nx + 2*ny - 8*D - 4 (size_NF2FF_total). */
+int32_t tomo_contour_len(const tomo_sim_config *cfg);
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct tomo_ctx tomo_ctx;
+
+/* Create a context on HIP device `device` with a device-memory budget for
+ * resident simulations (bytes; 0 = 90% of free memory). */
+int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx **out);
+int tomo_ctx_destroy(tomo_ctx *ctx);
+const char *tomo_last_error(void);
+
+/* Kernel schedule knobs (the autotuner writes these; 0 = keep current).
+ * tsteps: time steps fused per launch; seg_rows: rows per tile segment;
+ * waves: waves per workgroup; chunk: running-DFT fold interval in steps. */
+int tomo_set_schedule(tomo_ctx *ctx, int tsteps, int seg_rows, int waves, int chunk);
+int tomo_get_schedule(tomo_ctx *ctx, int *tsteps, int *seg_rows, int *waves, int *chunk);
+
+/* ---- evaluation -------------------------------------------------------- */
+/*
+ * One fitness evaluation (the FDTD_GPU() equivalent): rasterise the parameter
+ * vector into a phantom, run n_exc simulations one after another (one
+ * simulation per launch sequence: the "plain" path), project the far field
+ * and reduce the fitness in FP64 against `target` (n_target values laid out
+ * like D; NULL in physical mode selects the seeded target phantom).
+ * D_out / P_out (optional, tomo_farfield_len floats) receive the far field
+ * and its phase.
+ */
+int tomo_eval(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
+              size_t n_params, const float *target, size_t n_target,
+              double *fitness_out, float *D_out, float *P_out);
+
+/*
+ * Batched evaluation of n_vec parameter vectors (row-major, `dim` each) with
+ * a shared configuration: all n_vec * n_exc simulations advance together in
+ * time-blocked launches captured in a hipGraph.  Results equal tomo_eval's
+ * for each vector to within FP32 reduction-order tolerance.
+ */
+int tomo_eval_batch(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
+                    size_t n_vec, size_t dim, const float *target, size_t n_target,
+                    double *fitness_out, float *D_out);
+
+/* Far field of the seeded target phantom of a physical configuration. */
+int tomo_target_farfield(tomo_ctx *ctx, const tomo_sim_config *cfg, float *D_out);
+
+/* ---- continuous-batching request loop ---------------------------------- */
+/*
+ * tomo_submit queues one evaluation request (its own configuration, so
+ * requests of different grid sizes mix) and returns a ticket.  Requests are
+ * admitted into resident device slots when the HBM budget allows, advance in
+ * shared time-blocked epochs with everything else resident, and retire as
+ * soon as their last step is done.  tomo_progress runs one epoch (callers
+ * without a thread of their own); tomo_wait blocks until a ticket finishes
+ * (driving epochs itself).  Latency = submit -> completion, in seconds.
+ */
+int tomo_submit(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
+                size_t n_params, uint64_t *ticket_out);
+int tomo_poll(tomo_ctx *ctx, uint64_t ticket, double *fitness_out, double *latency_s);
+int tomo_wait(tomo_ctx *ctx, uint64_t ticket, double *fitness_out, double *latency_s);
+int tomo_cancel(tomo_ctx *ctx, uint64_t ticket);
+int tomo_progress(tomo_ctx *ctx, int *n_active_out);
+/* Resident device bytes one request of this configuration needs. */
+int64_t tomo_request_bytes(const tomo_sim_config *cfg);
+
+/* ---- single-simulation access (parity tests) --------------------------- */
+typedef struct tomo_sim tomo_sim;
+/* Build one simulation of excitation/antenna `exc` of the phantom `params`. */
+int tomo_sim_create(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
+                    size_t n_params, int exc, tomo_sim **out);
+int tomo_sim_destroy(tomo_sim *sim);
+/* Advance by nsteps (must keep the running DFT chunk boundaries consistent:
+ * the accumulated DFT is folded at every chunk boundary and at the end). */
+int tomo_sim_step(tomo_sim *sim, int nsteps);
+/* Copy fields to host: Ez (nx+1)*(ny+1), Hx and Hy nx*ny, row-major with the
+ * reference indexing x + y*(nx+1) and x + y*nx (FDTD_common.hxx:65). */
+int tomo_sim_fields(tomo_sim *sim, float *ez, float *hx, float *hy, float *jp);
+/* Material id map, (nx+1)*(ny+1) bytes, and the per-material coefficient
+ * table (n_mat x 8 floats: Ceze, Cezh, Cezeic, Cezeip, Cezjp, Kp, Beta_p/dt, 0). */
+int tomo_sim_materials(tomo_sim *sim, uint8_t *mat, float *coef, int *n_mat);
+/* Running-DFT accumulators after folding: J and M as interleaved complex
+ * [freq][point] (2*n_freq*P floats each), signs and dt applied as in
+ * calculate_JandM (FDTD_GPU.cu:231-280). */
+int tomo_sim_dft(tomo_sim *sim, float *J, float *M);
+/* Far field D and phase P ([freq][angle]) of this simulation. */
+int tomo_sim_farfield(tomo_sim *sim, float *D, float *P);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOMO_H */
