@@ -61,9 +61,10 @@ extern "C" {
 #define TOMO_FLAG_PARAMS_EPS_SIGMA 0x8 /* parameter layout {eps_r[n^2], sigma[n^2]} (FDTD_CPU.cxx:681-698) */
 #define TOMO_FLAG_PHASE_FITNESS 0x10  /* add the phase term sketched at FDTD_GPU.cu:1009-1018          */
 #define TOMO_FLAG_DFT_FP64 0x20       /* running-DFT accumulators in FP64 instead of Kahan FP32      */
+#define TOMO_FLAG_REL_FITNESS 0x40    /* relative misfit (D-m)^2/m^2 per term (FDTD_CPU.cxx:844)      */
 #define TOMO_FLAG_CPU_REF_VARIANT                                                              \
     (TOMO_FLAG_FWF_ABS_ORIGIN | TOMO_FLAG_CPU_COEFS | TOMO_FLAG_NO_NAN_CLAMP |             \
-     TOMO_FLAG_PARAMS_EPS_SIGMA)
+     TOMO_FLAG_PARAMS_EPS_SIGMA | TOMO_FLAG_REL_FITNESS)
 
 /* ---- presets ----------------------------------------------------------- */
 #define TOMO_PRESET_REF_GPU 0   /* FDTD_common.hxx macros + FDTD_GPU semantics, 27 parameters  */

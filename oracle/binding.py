@@ -33,6 +33,8 @@ class Oracle:
         L.orc_sim_destroy.argtypes = [_c.c_void_p]
         L.orc_sim_step.argtypes = [_c.c_void_p, _c.c_int]
         L.orc_sim_fields.argtypes = [_c.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_sim_probe.argtypes = [_c.c_void_p, _c.c_int, _c.c_int]
+        L.orc_sim_probe.restype = _c.c_double
         L.orc_sim_materials.argtypes = [_c.c_void_p, _c.POINTER(_c.c_uint8), _dp]
         L.orc_sim_cpml.argtypes = [_c.c_void_p, _dp]
         L.orc_sim_dft.argtypes = [_c.c_void_p, _dp, _dp]
@@ -96,6 +98,9 @@ class OracleSim:
 
     def step(self, n):
         self.o.lib.orc_sim_step(self.h, int(n))
+
+    def probe(self, x, y):
+        return self.o.lib.orc_sim_probe(self.h, int(x), int(y))
 
     def fields(self):
         c = self.cfg

@@ -65,20 +65,26 @@ struct Contour {
     std::vector<int> x, y, leg;  // leg: 0 yn, 1 xp, 2 yp, 3 xn
 };
 
-static Contour make_contour(int nx, int ny, int D) {
+// closed = false: the reference's map, whose x faces start one cell below the
+// bottom face (y = D) and whose y faces stop one cell short of the right face
+// (x = nx-D-2).  closed = true (physical mode): the same number of points on
+// a closed box: y faces x in [D+1, nx-D-1], x faces y in [D+1, ny-D-1].
+static Contour make_contour(int nx, int ny, int D, bool closed = false) {
     Contour c;
-    c.L1 = nx - 2 * D - 2;
-    c.L2 = nx - 4 * D + ny - 2;
-    c.L3 = 2 * nx - 6 * D + ny - 4;
-    c.P = 2 * nx - 8 * D + 2 * ny - 4;
+    const int xhi = closed ? nx - D - 1 : nx - D - 2;  // last x of the y faces
+    const int ylo = closed ? D + 1 : D;                 // first y of the x faces
+    c.L1 = xhi - D;
+    c.L2 = c.L1 + (ny - D - ylo);
+    c.L3 = c.L2 + (xhi - D);
+    c.P = c.L3 + (ny - D - ylo);
     for (int i = 0; i < c.P; ++i) {
         int x, y, g;
         if (i < c.L1) {
             x = i + D + 1; y = D; g = 0;
         } else if (i < c.L2) {
-            x = nx - D - 1; y = (i - c.L1) + D; g = 1;
+            x = nx - D - 1; y = (i - c.L1) + ylo; g = 1;
         } else if (i < c.L3) {
-            x = nx - D - (i - c.L2) - 2; y = ny - D - 1; g = 2;
+            x = xhi - (i - c.L2); y = ny - D - 1; g = 2;
         } else {
             x = D; y = ny - D - (i - c.L3) - 1; g = 3;
         }
@@ -446,11 +452,12 @@ static bool sim_init(Sim<R>& s, const tomo_sim_config& c, const double* p, size_
         return false;
     }
     if (s.ref && c.nx != c.ny) { g_err = "ref-compat mode requires nx == ny"; return false; }
-    s.ct = make_contour(c.nx, c.ny, c.ntff_offset);
+    s.ct = make_contour(c.nx, c.ny, c.ntff_offset, !s.ref);
     s.freq.resize(c.n_freq);
     for (int f = 0; f < c.n_freq; ++f) {
         double df = c.n_freq > 1 ? (c.f_stop - c.f_start) / (c.n_freq - 1) : 0.0;
         s.freq[f] = c.f_start + df * f;
+        if (s.ref) s.freq[f] = (double)(float)s.freq[f];  // `float freq` (FDTD_GPU.cu:698, 951)
     }
     if (s.ref) {
         if (!ref_materials(s, p, np)) return false;
@@ -857,7 +864,9 @@ static bool eval_full(const tomo_sim_config& c, const double* p, size_t np, cons
     }
     double fit = 0;
     const double norm = (double)c.n_exc * F;
-    for (size_t i = 0; i < L; ++i) fit -= (Dall[i] - tgt[i]) * (Dall[i] - tgt[i]) / norm;
+    const bool rel = (c.flags & TOMO_FLAG_REL_FITNESS) != 0;  // FDTD_CPU.cxx:844
+    for (size_t i = 0; i < L; ++i)
+        fit -= (Dall[i] - tgt[i]) * (Dall[i] - tgt[i]) / (rel ? tgt[i] * tgt[i] * norm : norm);
     if ((c.flags & TOMO_FLAG_PHASE_FITNESS) && target != nullptr && nt >= 2 * L) {
         for (size_t i = 0; i < L; ++i) {
             double dphi = Pall[i] - (double)target[L + i];
@@ -903,7 +912,7 @@ extern "C" {
 const char* orc_last_error(void) { return g_err.c_str(); }
 
 int orc_contour(const tomo_sim_config* c, int* xs, int* ys, int* legs) {
-    Contour ct = make_contour(c->nx, c->ny, c->ntff_offset);
+    Contour ct = make_contour(c->nx, c->ny, c->ntff_offset, c->mode == TOMO_MODE_PHYSICAL);
     for (int p = 0; p < ct.P; ++p) { xs[p] = ct.x[p]; ys[p] = ct.y[p]; legs[p] = ct.leg[p]; }
     return ct.P;
 }
@@ -923,6 +932,12 @@ int orc_sim_step(void* hp, int n) {
     Handle* h = (Handle*)hp;
     if (h->kind) sim_step(h->sd, n); else sim_step(h->sf, n);
     return 0;
+}
+
+// Ez at one node (the value FDTD_CPU.cxx:808 prints for the centre node).
+double orc_sim_probe(void* hp, int x, int y) {
+    Handle* h = (Handle*)hp;
+    return h->kind ? h->sd.Ez[h->sd.E(x, y)] : (double)h->sf.Ez[h->sf.E(x, y)];
 }
 
 int orc_sim_fields(void* hp, double* ez, double* hx, double* hy, double* jp) {

@@ -1,0 +1,76 @@
+"""The C-ABI library loads and exports every entry point include/tomo.h
+declares, plus the reference's C++ objective entry points; host-side logic
+that needs no GPU (presets, sizes, validation) behaves as specified."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import tomography_amd as t
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "tomo.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z0-9_]+\s*\*?\s*(tomo_[a-z0-9_]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = t.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(t.api.EXPORTS)
+
+
+def test_reference_entry_points_are_exported_with_cxx_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", t.api.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
+    # double FDTD_GPU(const std::vector<double>&) and FDTD_CPU (FDTD_GPU.cuh:8, FDTD_CPU.h:8)
+    assert "_Z8FDTD_GPURKSt6vectorIdSaIdEE" in out
+    assert "_Z8FDTD_CPURKSt6vectorIdSaIdEE" in out
+
+
+def test_presets_and_sizes():
+    for pid in (t.PRESET_CFG1, t.PRESET_CFG2, t.PRESET_CFG3):
+        c = t.preset(pid)
+        assert c.mode == t.MODE_PHYSICAL and c.ncpml == 10 and c.ntff_offset == 15
+        assert abs(c.dt - 0.99 * c.dx / (299792458.0 * np.sqrt(2))) / c.dt < 1e-3
+        assert t.lib().tomo_contour_len(ctypes.byref(c)) == 2 * c.nx + 2 * c.ny - 8 * 15 - 4
+    c3 = t.preset(t.PRESET_CFG3)
+    assert (c3.nx, c3.steps, c3.n_exc, c3.n_freq, c3.n_angles) == (512, 4096, 16, 16, 360)
+    c2 = t.preset(t.PRESET_CFG2)
+    assert (c2.nx, c2.steps, c2.dx) == (1024, 8192, 0.25e-3)
+    for n, steps in ((256, 2048), (512, 4096), (2048, 16384), (300, 2400)):
+        c = t.preset(t.PRESET_CFG4, n)
+        assert c.nx == n and c.steps == steps
+    # R0 = 1e-6 polynomial grading: sigma_max = -(m+1) ln R0 / (2 eta0 d)
+    eta0 = np.sqrt(np.pi * 4e-7 / 8.85418e-12)
+    assert abs(c3.sigma_max - 4 * np.log(1e6) / (2 * eta0 * 10 * 0.5e-3)) < 1e-9 * c3.sigma_max
+
+
+def test_request_bytes_scale_with_grid_and_antennas():
+    a = t.request_bytes(t.preset(t.PRESET_CFG4, 256))
+    b = t.request_bytes(t.preset(t.PRESET_CFG4, 512))
+    c = t.request_bytes(t.preset(t.PRESET_CFG4, 512).copy(n_exc=4))
+    assert 0 < a < b and c == 4 * b
+    assert 3.0 < b / a < 5.0  # dominated by the N^2 field planes
+    assert t.request_bytes(t.preset(t.PRESET_CFG4, 256).copy(ncpml=0)) == -1
+
+
+def test_invalid_preset_is_an_error():
+    with pytest.raises(t.TomoError):
+        t.preset(99)
+    with pytest.raises(t.TomoError):
+        t.preset(t.PRESET_CFG4, 10)
+
+
+def test_measurement_data_layout():
+    g = t.measurement("gpu")
+    c = t.measurement("cpu")
+    assert g.size == 8000 and c.size == 4000
+    assert np.all(g[:4000] >= 0) and np.all(np.abs(g[4000:]) <= np.pi)

@@ -1,0 +1,205 @@
+"""HIP path (libtomo.so through the C ABI) against the oracle.
+
+Tolerances (written here, stated in DESIGN.md):
+  * material maps, coefficient tables, contour indexing: bit-exact;
+  * fields after a few hundred steps: relative L2 <= 1e-5 against the FP32
+    oracle's restatement of the reference, <= 1e-4 against the FP64 oracle;
+  * running-DFT accumulators, far field and fitness: <= 1e-4 relative to FP64.
+"""
+import numpy as np
+import pytest
+
+import tomography_amd as t
+from cases import PHYS_PARAMS, phys_small, ref_params, ref_small, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL_F32 = 1e-5
+FIELD_TOL_F64 = 1e-4
+FF_TOL = 1e-4
+
+
+def oracle_fields(oracle, cfg, p, exc, steps, fp64):
+    s = oracle.sim(cfg, p, exc=exc, fp64=fp64)
+    s.step(steps)
+    return s
+
+
+def test_ref_materials_and_coefficients_bit_exact(ctx, oracle):
+    cfg = ref_small()
+    p = ref_params(11)
+    g = ctx.sim(cfg, p, exc=0)
+    mat, coef = g.materials()
+    om, oc = oracle.sim(cfg, p, exc=0).materials()
+    assert np.array_equal(mat, om)
+    for m in np.unique(mat):
+        sel = om == m
+        # oracle planes: Ceze, Cezhy, Cezhx, Cezeic, Cezeip, Cezjp, Beta_p, Beta_p/dt, Kp
+        want = np.array([oc[0][sel][0], oc[1][sel][0], oc[3][sel][0], oc[4][sel][0], oc[5][sel][0],
+                         oc[8][sel][0], oc[7][sel][0]], np.float32)
+        assert np.array_equal(coef[m, :7], want), m
+
+
+def test_phys_materials_bit_exact(ctx, oracle):
+    cfg = phys_small()
+    for pv in (PHYS_PARAMS, np.array([-6.0, 5.0, 5.5, 25.0, 40.0, 10.0, 0.5, 3.0, 4.0, 1.0, 2.5])):
+        g = ctx.sim(cfg, pv, exc=1)
+        mat, coef = g.materials()
+        om, oc = oracle.sim(cfg, pv, exc=1).materials()
+        assert np.array_equal(mat, om)
+        assert len(np.unique(mat)) >= 4
+        for m in np.unique(mat):
+            sel = om == m
+            want = np.array([oc[0][sel][0], oc[1][sel][0], oc[5][sel][0], oc[8][sel][0], oc[7][sel][0]],
+                            np.float32)
+            got = coef[m, [0, 1, 4, 5, 6]]
+            assert np.array_equal(got, want), (m, got, want)
+
+
+@pytest.mark.parametrize("exc", [0, 1])
+def test_ref_fields_vs_oracle(ctx, oracle, exc):
+    cfg = ref_small(steps=300)
+    p = ref_params(5)
+    g = ctx.sim(cfg, p, exc=exc)
+    g.step(300)
+    ez, hx, hy, jp = g.fields()
+    o32 = oracle_fields(oracle, cfg, p, exc, 300, False).fields()
+    o64 = oracle_fields(oracle, cfg, p, exc, 300, True).fields()
+    assert np.abs(o64[0]).max() > 1e-3  # the pulse has reached the phantom
+    for name, a, b32, b64 in zip(("ez", "hx", "hy", "jp"), (ez, hx, hy, jp), o32, o64):
+        assert rel_l2(a, b32) < FIELD_TOL_F32, (name, rel_l2(a, b32))
+        assert rel_l2(a, b64) < FIELD_TOL_F64, (name, rel_l2(a, b64))
+
+
+def test_phys_fields_vs_oracle(ctx, oracle):
+    cfg = phys_small(steps=400)
+    g = ctx.sim(cfg, PHYS_PARAMS, exc=1)
+    g.step(400)
+    got = g.fields()
+    o32 = oracle_fields(oracle, cfg, PHYS_PARAMS, 1, 400, False).fields()
+    o64 = oracle_fields(oracle, cfg, PHYS_PARAMS, 1, 400, True).fields()
+    assert np.abs(o64[0]).max() > 1e-3
+    for name, a, b32, b64 in zip(("ez", "hx", "hy", "jp"), got, o32, o64):
+        assert rel_l2(a, b32) < FIELD_TOL_F32, (name, rel_l2(a, b32))
+        assert rel_l2(a, b64) < FIELD_TOL_F64, (name, rel_l2(a, b64))
+
+
+@pytest.mark.parametrize("sched", [(1, 32, 8, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 8, 48), (8, 64, 2, 64)])
+def test_schedule_does_not_change_results(oracle, sched):
+    """Fused step count, tile height, waves per workgroup and DFT chunk only
+    reorder work: every schedule gives the same fields and accumulators."""
+    cfg = phys_small(steps=200)
+    base = t.Context()
+    base.set_schedule(4, 32, 8, 32)
+    a = base.sim(cfg, PHYS_PARAMS, exc=0)
+    a.step(200)
+    other = t.Context()
+    other.set_schedule(*sched)
+    b = other.sim(cfg, PHYS_PARAMS, exc=0)
+    b.step(90)
+    b.step(110)  # an odd split of the steps exercises partial launches and chunks
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(x, y)
+    Ja, Ma = a.dft()
+    Jb, Mb = b.dft()
+    assert rel_l2(Jb, Ja) < 1e-6 and rel_l2(Mb, Ma) < 1e-6
+    a.close(); b.close(); base.close(); other.close()
+
+
+def test_ref_dft_and_farfield_vs_oracle(ctx, oracle):
+    cfg = ref_small(steps=300)
+    p = ref_params(8)
+    g = ctx.sim(cfg, p, exc=0)
+    g.step(300)
+    J, M = g.dft()
+    o = oracle_fields(oracle, cfg, p, 0, 300, True)
+    Jo, Mo = o.dft()
+    assert rel_l2(J, Jo) < FF_TOL and rel_l2(M, Mo) < FF_TOL
+    D, P = g.farfield()
+    Do, Po = o.farfield()
+    assert rel_l2(D, Do) < FF_TOL
+
+
+def test_phys_dft_and_farfield_vs_oracle(ctx, oracle):
+    cfg = phys_small(steps=500)
+    g = ctx.sim(cfg, PHYS_PARAMS, exc=0)
+    g.step(500)
+    o = oracle_fields(oracle, cfg, PHYS_PARAMS, 0, 500, True)
+    J, M = g.dft()
+    Jo, Mo = o.dft()
+    assert rel_l2(J, Jo) < FF_TOL and rel_l2(M, Mo) < FF_TOL
+    D, _ = g.farfield()
+    Do, _ = o.farfield()
+    assert rel_l2(D, Do) < FF_TOL
+
+
+def test_ref_eval_fitness_vs_oracle(ctx, oracle):
+    cfg = ref_small(steps=300)
+    p = ref_params(21)
+    rng = np.random.default_rng(0)
+    target = rng.uniform(0.0, 0.05, cfg.farfield_len()).astype(np.float32)
+    fit, D, P = ctx.eval(cfg, p, target=target, want_farfield=True)
+    fo, Do, Po = oracle.eval(cfg, p, target=target, fp64=True)
+    assert rel_l2(D, Do) < FF_TOL
+    assert abs(fit - fo) <= FF_TOL * abs(fo)
+
+
+def test_phys_eval_fitness_vs_oracle(ctx, oracle):
+    cfg = phys_small(steps=400)
+    fit, D, _ = ctx.eval(cfg, PHYS_PARAMS, want_farfield=True)
+    fo, Do, _ = oracle.eval(cfg, PHYS_PARAMS, fp64=True)
+    assert rel_l2(D, Do) < FF_TOL
+    assert abs(fit - fo) <= FF_TOL * abs(fo)
+
+
+def test_batch_equals_plain(ctx):
+    cfg = phys_small(steps=300)
+    vecs = np.array([PHYS_PARAMS, PHYS_PARAMS + [1, 1, 0.5, 1, 2, 1, 0.1, 0], PHYS_PARAMS - [2, 0, 1, 2, 3, 2, 0.1, 0]])
+    fb, Db = ctx.eval_batch(cfg, vecs, want_farfield=True)
+    for i, v in enumerate(vecs):
+        fp, Dp, _ = ctx.eval(cfg, v, want_farfield=True)
+        assert np.array_equal(Db[i], Dp)
+        assert fb[i] == fp
+
+
+def test_requests_alone_equal_requests_in_mix(ctx):
+    """Continuous batching: a request's result does not depend on what else is
+    resident (different grids, antenna counts and admission times)."""
+    cfgs = [phys_small(steps=256, n=96, n_exc=1), phys_small(steps=320, n=128, n_exc=2),
+            phys_small(steps=200, n=160, n_exc=3)]
+    cfgs[0] = cfgs[0].copy(breast_radius=0.012, skin_thickness=0.001, src_radius=0.015)
+    vecs = [PHYS_PARAMS, PHYS_PARAMS + [1, -1, 0.5, 0, 0, 0, 0, 0], PHYS_PARAMS + [-2, 2, 1, 2, 2, 1, 0.1, 0]]
+    alone = [ctx.eval(c, v) for c, v in zip(cfgs, vecs)]
+    tickets = [ctx.submit(cfgs[0], vecs[0]), ctx.submit(cfgs[1], vecs[1])]
+    ctx.progress()
+    ctx.progress()
+    tickets.append(ctx.submit(cfgs[2], vecs[2]))  # admitted while the others are resident
+    got = [ctx.wait(tk) for tk in tickets]
+    for (f, lat), a in zip(got, alone):
+        assert f == a and lat > 0
+
+
+def test_error_paths(ctx):
+    cfg = ref_small()
+    with pytest.raises(t.TomoError) as e:
+        ctx.eval(cfg, np.ones(20), target=np.zeros(cfg.farfield_len(), np.float32))
+    assert e.value.code == t.TOMO_E_SHORT
+    bad = ref_params(1)
+    bad[0] = -1.0
+    with pytest.raises(t.TomoError) as e:
+        ctx.eval(cfg, bad, target=np.zeros(cfg.farfield_len(), np.float32))
+    assert e.value.code == t.TOMO_E_INVALID
+    pc = phys_small()
+    with pytest.raises(t.TomoError):
+        ctx.submit(pc, np.array([0, 0, 4, 20, 34, np.nan, 0, 0.0]))
+    small = t.Context(hbm_budget=1 << 20)
+    tk = small.submit(pc, PHYS_PARAMS)
+    with pytest.raises(t.TomoError) as e:
+        small.wait(tk)
+    assert e.value.code == t.TOMO_E_NOMEM
+    tk2 = ctx.submit(pc, PHYS_PARAMS)
+    ctx.cancel(tk2)
+    with pytest.raises(t.TomoError) as e:
+        ctx.poll(tk2)
+    assert e.value.code == t.TOMO_E_CANCELLED
+    small.close()

@@ -1,0 +1,40 @@
+// kernels.h — host-callable launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "setup.h"
+#include "tomo_internal.h"
+
+namespace tomo {
+
+// Steps fused per launch that have a compiled kernel.
+bool tblock_supported(int T);
+
+// Time-blocked E/H update of every tile in `tiles` (T steps, one launch).
+hipError_t launch_tblock(int T, int mode, const TileItem* tiles, int ntiles, const LaunchItem* items,
+                         const DevSim* sims, int ordinal, int seg_rows, int waves, int grid, hipStream_t st);
+
+// Fold one chunk of surface samples into the running DFT (Kahan FP32 or FP64).
+hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
+                       hipStream_t st);
+
+// Far-field projection of the running DFT: D and phase per (sim, frequency).
+hipError_t launch_project(const ProjItem* items, int nitems, const DevSim* sims, const float* const* pos,
+                          int n_angles, float eta0, float* D, float* Pph, hipStream_t st);
+
+// FP64 fitness reduction, one block per evaluation:
+// fit[e] = -sum_i (D[e*L+i] - tgt[i])^2 / norm  (+ optional wrapped phase term).
+// terms bit 0: phase term; bit 1: relative misfit (divide each term by tgt^2).
+hipError_t launch_fitness(const float* D, const float* Pph, const float* tgt, const float* tgt_phase, int L,
+                          int n_eval, double norm, int terms, double* fit, hipStream_t st);
+
+// Material rasterisation (bit-exact with the oracle).
+hipError_t launch_raster_ref(uint8_t* mat, int pitch, int nx, int ny, const int* lo, const int* hi,
+                             const int* xc, const int* yc, int npatch, hipStream_t st);
+hipError_t launch_raster_phys(uint8_t* mat, int pitch, int nx, int ny, float dxf, float dyf,
+                              const PhantomGeom& g, hipStream_t st);
+
+// Zero-fill (a kernel, so that it is safe inside captured graphs).
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
+
+}  // namespace tomo

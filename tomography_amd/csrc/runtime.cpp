@@ -1,0 +1,984 @@
+// runtime.cpp — context, resident simulation slots, the epoch runner and the
+// C ABI of include/tomo.h.
+//
+// An epoch advances every active simulation by up to `chunk` steps:
+// ceil(chunk / T) launches of the time-blocked update (T steps each, one tile
+// list spanning all simulations) followed by one running-DFT fold.  Plain
+// evaluation (tomo_eval, the FDTD_GPU() twin at FDTD_GPU.cu:650-1110) runs one
+// simulation at a time; batched evaluation and the request loop put every
+// resident simulation into the same epochs.  Steady-state epochs of the
+// batched path are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.h"
+#include "setup.h"
+
+using namespace tomo;
+
+namespace {
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(TOMO_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct Slot {
+    SimSetup su;
+    DevSim d{};
+    char* mem = nullptr;
+    size_t bytes = 0;
+    int idx = -1;       // DevSim table index
+    int done = 0;       // steps completed
+    int parity = 0;     // buffer holding the current state
+    int target = 0;     // steps to reach in the current run
+    long req = -1;      // owning request (serving)
+    int exc = 0;
+};
+
+// Bytes one simulation slot occupies on the device.
+size_t slot_bytes(const SimSetup& s, int chunk) {
+    const tomo_sim_config& c = s.cfg;
+    const size_t rows = (size_t)c.ny + 1, pitch = s.pitch, nc = c.ncpml, P = s.P;
+    size_t b = 0;
+    b += 6 * align256(rows * pitch * 4);                      // ez, hx, hy x2
+    b += 2 * align256((size_t)std::max(s.jp_len, 1) * 4);     // jp x2
+    b += 4 * align256(rows * 2 * nc * 4);                     // pex, phx x2
+    b += 4 * align256(2 * nc * pitch * 4);                    // pey, phy x2
+    b += align256(rows * pitch);                              // mat
+    b += align256(3 * rows * 4);                              // span
+    b += align256(3 * (size_t)chunk * P * 4);                 // samples
+    b += 2 * align256((size_t)c.n_freq * P * 16);             // accJ, accM
+    b += align256((size_t)std::max(c.steps, 1) * 4);          // source waveform
+    b += align256(2 * P * 4);                                 // contour positions
+    b += align256((2 * rows + (size_t)c.nx + 1 + rows) * 4);  // ref raster inputs
+    return b;
+}
+
+}  // namespace
+
+struct tomo_request {
+    uint64_t ticket = 0;
+    tomo_sim_config cfg{};
+    std::vector<double> params;
+    std::vector<Slot*> slots;
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed, 4 cancelled
+    int status = TOMO_OK;
+    double t_submit = 0, t_done = 0, fitness = 0;
+    int64_t bytes = 0;
+};
+
+struct tomo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    uint64_t budget = 0, used = 0;
+    int T = 4, S = 32, W = 8, K = 32;  // schedule
+    int grid_mult = 4;                 // persistent workgroups per CU
+    bool use_graph = true;
+    // DevSim table
+    DevSim* d_sims = nullptr;
+    int cap = 0;
+    std::vector<int> free_idx;
+    // epoch staging
+    LaunchItem* h_items = nullptr;
+    LaunchItem* d_items = nullptr;
+    int items_cap = 0;
+    TileItem* h_tiles = nullptr;
+    TileItem* d_tiles = nullptr;
+    int tiles_cap = 0;
+    hipEvent_t staged = nullptr;
+    std::vector<int> tile_key;  // slot table indices the uploaded tile list was built for
+    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0;
+    int ntiles = 0;
+    // serving
+    std::mutex mu;
+    uint64_t next_ticket = 1;
+    std::deque<uint64_t> queue;
+    std::map<uint64_t, std::unique_ptr<tomo_request>> reqs;
+    std::vector<Slot*> active;
+    // physical-mode targets, keyed by configuration bytes
+    std::map<std::string, std::vector<float>> targets;
+};
+
+namespace {
+
+std::string cfg_key(const tomo_sim_config& c) {
+    tomo_sim_config k = c;
+    k.n_params = 0;
+    return std::string((const char*)&k, sizeof(k));
+}
+
+int ensure_table(tomo_ctx* c, int need) {
+    if (need <= c->cap) return TOMO_OK;
+    int ncap = std::max(need, std::max(256, c->cap * 2));
+    DevSim* nd = nullptr;
+    HIPCHK(hipMalloc(&nd, sizeof(DevSim) * ncap));
+    if (c->d_sims) {
+        HIPCHK(hipMemcpyAsync(nd, c->d_sims, sizeof(DevSim) * c->cap, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->d_sims));
+    }
+    for (int i = ncap - 1; i >= c->cap; --i) c->free_idx.push_back(i);
+    c->d_sims = nd;
+    c->cap = ncap;
+    return TOMO_OK;
+}
+
+int ensure_staging(tomo_ctx* c, int nitems, int ntiles) {
+    if (nitems > c->items_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_items) HIPCHK(hipHostFree(c->h_items));
+        if (c->d_items) HIPCHK(hipFree(c->d_items));
+        int cap = std::max(nitems, 2 * c->items_cap);
+        HIPCHK(hipHostMalloc(&c->h_items, sizeof(LaunchItem) * cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&c->d_items, sizeof(LaunchItem) * cap));
+        c->items_cap = cap;
+    }
+    if (ntiles > c->tiles_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_tiles) HIPCHK(hipHostFree(c->h_tiles));
+        if (c->d_tiles) HIPCHK(hipFree(c->d_tiles));
+        int cap = std::max(ntiles, 2 * c->tiles_cap);
+        HIPCHK(hipHostMalloc(&c->h_tiles, sizeof(TileItem) * cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&c->d_tiles, sizeof(TileItem) * cap));
+        c->tiles_cap = cap;
+        c->tile_key.clear();
+    }
+    return TOMO_OK;
+}
+
+// Build the device-resident simulation for one excitation of one phantom.
+int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t np, int exc, Slot** out) {
+    std::unique_ptr<Slot> s(new Slot());
+    int rc = build_setup(cfg, p, np, exc, s->su);
+    if (rc) return rc;
+    const SimSetup& su = s->su;
+    if (c->K % c->T) return fail(TOMO_E_INVALID, "chunk must be a multiple of the fused step count");
+    const size_t bytes = slot_bytes(su, c->K);
+    if (c->used + bytes > c->budget) return fail(TOMO_E_NOMEM, "HBM budget exceeded");
+    HIPCHK(hipMalloc(&s->mem, bytes));
+    s->bytes = bytes;
+    c->used += bytes;
+    rc = ensure_table(c, 1);
+    if (rc) return rc;
+    if (c->free_idx.empty()) {
+        rc = ensure_table(c, c->cap + 1);
+        if (rc) return rc;
+    }
+    s->idx = c->free_idx.back();
+    c->free_idx.pop_back();
+
+    const tomo_sim_config& cf = su.cfg;
+    const size_t rows = (size_t)cf.ny + 1, pitch = su.pitch, nc = cf.ncpml, P = su.P;
+    char* m = s->mem;
+    auto take = [&](size_t b) { char* r = m; m += align256(b); return r; };
+    DevSim& d = s->d;
+    for (int k = 0; k < 2; ++k) d.ez[k] = (float*)take(rows * pitch * 4);
+    for (int k = 0; k < 2; ++k) d.hx[k] = (float*)take(rows * pitch * 4);
+    for (int k = 0; k < 2; ++k) d.hy[k] = (float*)take(rows * pitch * 4);
+    for (int k = 0; k < 2; ++k) d.jp[k] = (float*)take((size_t)std::max(su.jp_len, 1) * 4);
+    for (int k = 0; k < 2; ++k) d.pex[k] = (float*)take(rows * 2 * nc * 4);
+    for (int k = 0; k < 2; ++k) d.phx[k] = (float*)take(rows * 2 * nc * 4);
+    for (int k = 0; k < 2; ++k) d.pey[k] = (float*)take(2 * nc * pitch * 4);
+    for (int k = 0; k < 2; ++k) d.phy[k] = (float*)take(2 * nc * pitch * 4);
+    char* state_end = m;
+    uint8_t* mat = (uint8_t*)take(rows * pitch);
+    int* span = (int*)take(3 * rows * 4);
+    d.samp = (float*)take(3 * (size_t)c->K * P * 4);
+    char* acc0 = m;
+    d.accJ = take((size_t)cf.n_freq * P * 16);
+    d.accM = take((size_t)cf.n_freq * P * 16);
+    char* acc1 = m;
+    float* srcw = (float*)take((size_t)std::max(cf.steps, 1) * 4);
+    float* pos = (float*)take(2 * P * 4);
+    int* rr = (int*)take((2 * rows + (size_t)cf.nx + 1 + rows) * 4);
+    if ((size_t)(m - s->mem) > bytes) return fail(TOMO_E_INVALID, "internal: slot layout overflow");
+    d.mat = mat;
+    d.span = span;
+    d.srcw = cf.mode == TOMO_MODE_PHYSICAL ? srcw : nullptr;
+    d.pos = pos;
+    d.nx = cf.nx; d.ny = cf.ny; d.pitch = su.pitch; d.nc = cf.ncpml;
+    d.D = cf.ntff_offset; d.P = su.P; d.L1 = su.L1; d.L2 = su.L2; d.L3 = su.L3;
+    d.cx_hi = su.cx_hi; d.cy_lo = su.cy_lo;
+    d.sx = su.sx; d.sy = su.sy; d.mode = cf.mode; d.flags = cf.flags;
+    d.nmat = su.nmat; d.nfreq = cf.n_freq; d.chunk = c->K; d.steps = cf.steps;
+    std::memcpy(d.coef, su.coef, sizeof(d.coef));
+    for (int i = 0; i < TOMO_MAX_NC; ++i) {
+        d.ike[i] = su.ike[i]; d.ae[i] = su.ae[i]; d.be[i] = su.be[i];
+        d.ikm[i] = su.ikm[i]; d.am[i] = su.am[i]; d.bm[i] = su.bm[i];
+        d.ke[i] = su.ke[i]; d.km[i] = su.km[i];
+    }
+    d.chx = su.chx; d.chy = su.chy; d.chx_d = su.chx_d; d.chy_d = su.chy_d;
+    d.dxf = (float)cf.dx; d.dyf = (float)cf.dy;
+    d.inc_dt = su.inc_dt; d.inc_c = su.inc_c; d.inc_ax = su.inc_ax; d.inc_ay = su.inc_ay;
+    d.inc_itau = su.inc_itau; d.inc_abs_origin = su.inc_abs;
+    for (int f = 0; f < TOMO_MAX_FREQ; ++f) d.wdt[f] = su.wdt[f];
+    d.tJ = su.tJ; d.tM = su.tM;
+    const float dtf = (float)cf.dt;
+    // calculate_JandM signs (FDTD_GPU.cu:251-276): yn +, xp +, yp -, xn -
+    d.sgn_dt[0] = dtf; d.sgn_dt[1] = dtf; d.sgn_dt[2] = -dtf; d.sgn_dt[3] = -dtf;
+
+    // contour positions (N2FPostProcess geometry, FDTD_GPU.cu:584-621)
+    std::vector<float> hpos(2 * P);
+    {
+        const int D = cf.ntff_offset, nx = cf.nx, ny = cf.ny;
+        for (int p = 0; p < (int)P; ++p) {
+            int x, y, leg;
+            if (p < su.L1) { x = p + D + 1; y = D; leg = 0; }
+            else if (p < su.L2) { x = nx - D - 1; y = (p - su.L1) + su.cy_lo; leg = 1; }
+            else if (p < su.L3) { x = su.cx_hi - (p - su.L2); y = ny - D - 1; leg = 2; }
+            else { x = D; y = ny - D - (p - su.L3) - 1; leg = 3; }
+            double fx = x, fy = y;
+            if (leg == 0 || leg == 2) fy += 0.5; else fx += 0.5;
+            double px, py;
+            if (cf.mode == TOMO_MODE_REF_COMPAT) {
+                px = cf.dx * (-(double)(float)nx / 2 + 1 + fx);
+                py = cf.dy * (-(double)(float)ny / 2 + 1 + fy);
+            } else {
+                px = cf.dx * (fx - nx / 2.0);
+                py = cf.dy * (fy - ny / 2.0);
+            }
+            hpos[p] = (float)px;
+            hpos[P + p] = (float)py;
+        }
+    }
+    hipStream_t st = c->stream;
+    // zero the dynamic state and accumulators
+    HIPCHK(launch_zero(s->mem, (size_t)(state_end - s->mem), st));
+    HIPCHK(launch_zero(acc0, (size_t)(acc1 - acc0), st));
+    HIPCHK(hipMemcpyAsync(span, su.span.data(), 3 * rows * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(pos, hpos.data(), 2 * P * 4, hipMemcpyHostToDevice, st));
+    if (cf.mode == TOMO_MODE_PHYSICAL) {
+        HIPCHK(hipMemcpyAsync(srcw, su.srcw.data(), (size_t)cf.steps * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(launch_raster_phys(mat, su.pitch, cf.nx, cf.ny, (float)cf.dx, (float)cf.dy, su.geom, st));
+    } else {
+        int* lo = rr;
+        int* hi = rr + rows;
+        int* xc = rr + 2 * rows;
+        int* yc = xc + cf.nx + 1;
+        HIPCHK(hipMemcpyAsync(lo, su.ref.lo.data(), rows * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(hi, su.ref.hi.data(), rows * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(xc, su.ref.xc.data(), ((size_t)cf.nx + 1) * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(yc, su.ref.yc.data(), rows * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(launch_raster_ref(mat, su.pitch, cf.nx, cf.ny, lo, hi, xc, yc, cf.numpatches, st));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_sims + s->idx, &d, sizeof(DevSim), hipMemcpyHostToDevice, st));
+    // the host copies above read pageable memory owned by `s`/`hpos`: wait before they go away
+    HIPCHK(hipStreamSynchronize(st));
+    s->exc = exc;
+    s->target = cf.steps;
+    *out = s.release();
+    return TOMO_OK;
+}
+
+void slot_destroy(tomo_ctx* c, Slot* s) {
+    if (!s) return;
+    if (s->mem) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(s->mem);
+        c->used -= s->bytes;
+    }
+    if (s->idx >= 0) c->free_idx.push_back(s->idx);
+    delete s;
+}
+
+int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); }
+
+// Run epochs until every slot in `sl` reaches its target step.
+int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
+    const int T = c->T, S = c->S, W = c->W, K = c->K;
+    if (!tblock_supported(T) || K % T || K > 64 || W < 1 || W > 8 || S < 1)
+        return fail(TOMO_E_INVALID, "unsupported schedule");
+    for (Slot* s : sl)
+        if (s->d.chunk != K) return fail(TOMO_E_STATE, "simulation was created under a different DFT chunk");
+    for (int mode = 0; mode < 2; ++mode) {
+        std::vector<Slot*> ms;
+        for (Slot* s : sl)
+            if (s->d.mode == mode && s->done < s->target) ms.push_back(s);
+        bool fp64_any = false, fp32_any = false;
+        for (Slot* s : ms) (s->d.flags & TOMO_FLAG_DFT_FP64) ? fp64_any = true : fp32_any = true;
+        while (!ms.empty()) {
+            // tile list (rebuilt only when the set of simulations changes)
+            std::vector<int> key;
+            for (Slot* s : ms) key.push_back(s->idx);
+            int ntiles = 0;
+            for (Slot* s : ms) {
+                const int nb = n_bands(s->d.nx, T), ng = (nb + W - 1) / W, ns = (s->d.ny + 1 + S - 1) / S;
+                ntiles += ng * ns;
+            }
+            int rc = ensure_staging(c, (int)ms.size(), ntiles);
+            if (rc) return rc;
+            HIPCHK(hipEventSynchronize(c->staged));
+            int maxP = 0, maxn = 0;
+            for (size_t i = 0; i < ms.size(); ++i) {
+                Slot* s = ms[i];
+                LaunchItem& it = c->h_items[i];
+                it.sim = s->idx;
+                it.step0 = s->done;
+                it.parity0 = s->parity;
+                it.nsteps = std::min(K, s->target - s->done);
+                maxP = std::max(maxP, s->d.P);
+                maxn = std::max(maxn, it.nsteps);
+            }
+            HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * ms.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+            if (key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S || c->tile_key_W != W) {
+                int t = 0;
+                for (size_t i = 0; i < ms.size(); ++i) {
+                    Slot* s = ms[i];
+                    const int nb = n_bands(s->d.nx, T), ng = (nb + W - 1) / W, ns = (s->d.ny + 1 + S - 1) / S;
+                    for (int g = 0; g < ns; ++g)
+                        for (int b = 0; b < ng; ++b) c->h_tiles[t++] = TileItem{(int)i, b * W, g, 0};
+                }
+                HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, sizeof(TileItem) * ntiles, hipMemcpyHostToDevice,
+                                      c->stream));
+                c->tile_key = key;
+                c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W;
+                c->ntiles = ntiles;
+            }
+            HIPCHK(hipEventRecord(c->staged, c->stream));
+            const int grid = std::max(1, std::min(ntiles, c->cus * c->grid_mult));
+            const int nl = (maxn + T - 1) / T;
+            for (int o = 0; o < nl; ++o)
+                HIPCHK(launch_tblock(T, mode, c->d_tiles, ntiles, c->d_items, c->d_sims, o, S, W, grid, c->stream));
+            if (fp32_any) HIPCHK(launch_fold(false, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+            if (fp64_any) HIPCHK(launch_fold(true, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+            std::vector<Slot*> next;
+            for (size_t i = 0; i < ms.size(); ++i) {
+                Slot* s = ms[i];
+                const int n = c->h_items[i].nsteps;
+                s->done += n;
+                s->parity ^= ((n + T - 1) / T) & 1;
+                if (s->done < s->target) next.push_back(s);
+            }
+            ms.swap(next);
+        }
+    }
+    return TOMO_OK;
+}
+
+// fold kernels of both precisions skip the items of the other precision
+}  // namespace
+
+// The fold kernel template is chosen per slot precision; items of the other
+// precision are harmless because both layouts are 16 bytes per entry, but the
+// arithmetic would be wrong, so keep one precision per run.
+static int check_uniform_precision(const std::vector<Slot*>& sl) {
+    int f64 = -1;
+    for (Slot* s : sl) {
+        int v = (s->d.flags & TOMO_FLAG_DFT_FP64) ? 1 : 0;
+        if (f64 >= 0 && v != f64) return fail(TOMO_E_INVALID, "mixed DFT precision in one run");
+        f64 = v;
+    }
+    return TOMO_OK;
+}
+
+// Project the far field of `sl` into D/P (device, [slot][freq][angle] at
+// offset out_off[i]).
+static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<int>& out_off, float* dD,
+                   float* dP) {
+    if (sl.empty()) return TOMO_OK;
+    std::vector<ProjItem> items;
+    int A = sl[0]->su.cfg.n_angles;
+    for (size_t i = 0; i < sl.size(); ++i) {
+        const SimSetup& su = sl[i]->su;
+        if (su.cfg.n_angles != A) return fail(TOMO_E_INVALID, "mixed angle counts in one projection");
+        for (int f = 0; f < su.cfg.n_freq; ++f) {
+            ProjItem p{};
+            p.sim = sl[i]->idx;
+            p.f = f;
+            p.out = out_off[i] + f * A;
+            p.k = (float)(2 * kPI * su.freq[f] / c0());
+            p.inv_norm = (float)su.inv_norm[f];
+            items.push_back(p);
+        }
+    }
+    ProjItem* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(ProjItem) * items.size()));
+    HIPCHK(hipMemcpyAsync(d, items.data(), sizeof(ProjItem) * items.size(), hipMemcpyHostToDevice, c->stream));
+    hipError_t e = launch_project(d, (int)items.size(), c->d_sims, nullptr, A, (float)eta0(), dD, dP, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(d));
+    HIPCHK(e);
+    return TOMO_OK;
+}
+
+// Run the given simulations (all of one configuration) to completion and
+// project them; D_dev receives [slot][freq][angle].
+static int run_and_project(tomo_ctx* c, const std::vector<Slot*>& sl, float* dD, float* dP) {
+    int rc = check_uniform_precision(sl);
+    if (rc) return rc;
+    rc = run_epochs(c, sl);
+    if (rc) return rc;
+    std::vector<int> off;
+    for (size_t i = 0; i < sl.size(); ++i) off.push_back((int)(i * sl[i]->su.cfg.n_freq * sl[i]->su.cfg.n_angles));
+    return project(c, sl, off, dD, dP);
+}
+
+static int get_target(tomo_ctx* c, const tomo_sim_config& cfg, std::vector<float>& out);
+
+// Evaluate n_vec vectors; `plain` runs one simulation at a time.
+static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* params, size_t n_vec, size_t dim,
+                     const float* target, size_t n_target, double* fit_out, float* D_out, float* P_out, bool plain) {
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!params && n_vec) return fail(TOMO_E_INVALID, "null parameters");
+    const int F = cfg.n_freq, A = cfg.n_angles, E = cfg.n_exc;
+    const size_t L = (size_t)E * F * A;
+    std::vector<float> tgt;
+    const float* tph = nullptr;
+    if (target) {
+        if (n_target < L) return fail(TOMO_E_INVALID, "target shorter than the far-field record");
+        tgt.assign(target, target + L);
+        if ((cfg.flags & TOMO_FLAG_PHASE_FITNESS) && n_target >= 2 * L) tph = target + L;
+    } else {
+        if (cfg.mode != TOMO_MODE_PHYSICAL) return fail(TOMO_E_INVALID, "ref-compat mode needs a target (measurement)");
+        rc = get_target(c, cfg, tgt);
+        if (rc) return rc;
+    }
+    float *dD = nullptr, *dP = nullptr, *dT = nullptr;
+    double* dF = nullptr;
+    const size_t nrec = n_vec * L;
+    HIPCHK(hipMalloc(&dD, nrec * 4 + 256));
+    HIPCHK(hipMalloc(&dP, nrec * 4 + 256));
+    HIPCHK(hipMalloc(&dT, 2 * L * 4 + 256));
+    HIPCHK(hipMalloc(&dF, n_vec * 8 + 256));
+    HIPCHK(hipMemcpyAsync(dT, tgt.data(), L * 4, hipMemcpyHostToDevice, c->stream));
+    if (tph) HIPCHK(hipMemcpyAsync(dT + L, tph, L * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    rc = TOMO_OK;
+    if (plain) {
+        for (size_t v = 0; v < n_vec && !rc; ++v) {
+            for (int e = 0; e < E && !rc; ++e) {
+                Slot* s = nullptr;
+                rc = slot_create(c, cfg, params + v * dim, dim, e, &s);
+                if (rc) break;
+                std::vector<Slot*> one{s};
+                rc = run_and_project(c, one, dD + v * L + (size_t)e * F * A, dP + v * L + (size_t)e * F * A);
+                slot_destroy(c, s);
+            }
+        }
+    } else {
+        // as many whole vectors per pass as the HBM budget admits
+        size_t v0 = 0;
+        while (v0 < n_vec && !rc) {
+            std::vector<Slot*> sl;
+            size_t v = v0;
+            for (; v < n_vec && !rc; ++v) {
+                std::vector<Slot*> mine;
+                for (int e = 0; e < E; ++e) {
+                    Slot* s = nullptr;
+                    int r2 = slot_create(c, cfg, params + v * dim, dim, e, &s);
+                    if (r2) {
+                        for (Slot* m : mine) slot_destroy(c, m);
+                        if (r2 == TOMO_E_NOMEM && !sl.empty()) goto run_pass;
+                        rc = r2;
+                        break;
+                    }
+                    mine.push_back(s);
+                }
+                if (rc) break;
+                sl.insert(sl.end(), mine.begin(), mine.end());
+            }
+        run_pass:
+            if (!rc) rc = run_and_project(c, sl, dD + v0 * L, dP + v0 * L);
+            for (Slot* s : sl) slot_destroy(c, s);
+            v0 = v;
+        }
+    }
+    if (!rc) {
+        const int terms = (tph ? 1 : 0) | ((cfg.flags & TOMO_FLAG_REL_FITNESS) ? 2 : 0);
+        hipError_t e = launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, (int)n_vec, (double)E * F,
+                                      terms, dF, c->stream);
+        if (e != hipSuccess) rc = fail(TOMO_E_HIP, hipGetErrorString(e));
+    }
+    if (!rc) {
+        std::vector<double> hf(n_vec);
+        hipError_t e = hipMemcpyAsync(hf.data(), dF, n_vec * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && D_out) e = hipMemcpyAsync(D_out, dD, nrec * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && P_out) e = hipMemcpyAsync(P_out, dP, nrec * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(TOMO_E_HIP, hipGetErrorString(e));
+        for (size_t v = 0; v < n_vec && !rc; ++v) {
+            if (cfg.mode == TOMO_MODE_PHYSICAL && !std::isfinite(hf[v])) rc = fail(TOMO_E_NONFINITE, "non-finite fitness");
+            if (fit_out) fit_out[v] = hf[v];
+        }
+    }
+    (void)hipFree(dD);
+    (void)hipFree(dP);
+    (void)hipFree(dT);
+    (void)hipFree(dF);
+    return rc;
+}
+
+static int get_target(tomo_ctx* c, const tomo_sim_config& cfg, std::vector<float>& out) {
+    const std::string key = cfg_key(cfg);
+    auto it = c->targets.find(key);
+    if (it != c->targets.end()) {
+        out = it->second;
+        return TOMO_OK;
+    }
+    const std::vector<double> tv = target_vector(cfg);
+    const size_t L = (size_t)cfg.n_exc * cfg.n_freq * cfg.n_angles;
+    std::vector<Slot*> sl;
+    int rc = TOMO_OK;
+    for (int e = 0; e < cfg.n_exc && !rc; ++e) {
+        Slot* s = nullptr;
+        rc = slot_create(c, cfg, tv.data(), tv.size(), e, &s);
+        if (!rc) sl.push_back(s);
+    }
+    float *dD = nullptr, *dP = nullptr;
+    if (!rc && hipMalloc(&dD, L * 4) != hipSuccess) rc = fail(TOMO_E_HIP, "hipMalloc");
+    if (!rc && hipMalloc(&dP, L * 4) != hipSuccess) rc = fail(TOMO_E_HIP, "hipMalloc");
+    if (!rc) rc = run_and_project(c, sl, dD, dP);
+    if (!rc) {
+        out.resize(L);
+        if (hipMemcpy(out.data(), dD, L * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(TOMO_E_HIP, "copy");
+    }
+    for (Slot* s : sl) slot_destroy(c, s);
+    if (dD) (void)hipFree(dD);
+    if (dP) (void)hipFree(dP);
+    if (!rc) c->targets[key] = out;
+    return rc;
+}
+
+// ---------------------------------------------------------------- serving
+static int64_t request_bytes(const tomo_sim_config& cfg, int chunk) {
+    if (validate(cfg)) return -1;
+    SimSetup su;
+    su.cfg = cfg;
+    su.pitch = ((cfg.nx + 1) + 31) & ~31;
+    su.P = contour_len(cfg);
+    // upper bound of the compact Jp storage: the dispersive disc's bounding box
+    const double R = cfg.breast_radius;
+    const int rc = std::min(cfg.nx + 1, (int)std::ceil(2 * R / cfg.dx) + 3);
+    su.jp_len = rc * std::min(cfg.ny + 1, (int)std::ceil(2 * R / cfg.dy) + 3);
+    return (int64_t)(slot_bytes(su, chunk) * (size_t)cfg.n_exc);
+}
+
+static int finish_request(tomo_ctx* c, tomo_request* r) {
+    const tomo_sim_config& cfg = r->cfg;
+    const size_t L = (size_t)cfg.n_exc * cfg.n_freq * cfg.n_angles;
+    std::vector<float> tgt;
+    int rc = TOMO_OK;
+    if (cfg.mode == TOMO_MODE_PHYSICAL) rc = get_target(c, cfg, tgt);
+    else rc = fail(TOMO_E_INVALID, "serving needs physical mode");
+    float *dD = nullptr, *dP = nullptr, *dT = nullptr;
+    double* dF = nullptr;
+    if (!rc && (hipMalloc(&dD, L * 4) || hipMalloc(&dP, L * 4) || hipMalloc(&dT, L * 4) || hipMalloc(&dF, 256)))
+        rc = fail(TOMO_E_HIP, "hipMalloc");
+    if (!rc) {
+        std::vector<int> off;
+        for (size_t i = 0; i < r->slots.size(); ++i) off.push_back((int)(i * cfg.n_freq * cfg.n_angles));
+        rc = project(c, r->slots, off, dD, dP);
+    }
+    double fit = 0;
+    if (!rc) {
+        if (hipMemcpyAsync(dT, tgt.data(), L * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            launch_fitness(dD, dP, dT, nullptr, (int)L, 1, (double)cfg.n_exc * cfg.n_freq, 0, dF, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(&fit, dF, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(TOMO_E_HIP, "fitness");
+    }
+    if (dD) (void)hipFree(dD);
+    if (dP) (void)hipFree(dP);
+    if (dT) (void)hipFree(dT);
+    if (dF) (void)hipFree(dF);
+    for (Slot* s : r->slots) slot_destroy(c, s);
+    r->slots.clear();
+    r->fitness = fit;
+    r->status = rc ? rc : (std::isfinite(fit) ? TOMO_OK : TOMO_E_NONFINITE);
+    r->state = r->status == TOMO_OK ? 2 : 3;
+    r->t_done = now_s();
+    return rc;
+}
+
+// One epoch of the request loop: admit, advance every resident simulation by
+// one chunk, retire finished requests.
+static int progress_once(tomo_ctx* c, int* n_active) {
+    // admission by HBM footprint, FIFO
+    while (!c->queue.empty()) {
+        tomo_request* r = c->reqs[c->queue.front()].get();
+        if (r->state == 4) { c->queue.pop_front(); continue; }
+        if (c->used + (uint64_t)r->bytes > c->budget) {
+            if (c->active.empty()) {  // cannot ever fit
+                r->state = 3;
+                r->status = fail(TOMO_E_NOMEM, "request larger than the HBM budget");
+                r->t_done = now_s();
+                c->queue.pop_front();
+                continue;
+            }
+            break;
+        }
+        c->queue.pop_front();
+        int rc = TOMO_OK;
+        for (int e = 0; e < r->cfg.n_exc && !rc; ++e) {
+            Slot* s = nullptr;
+            rc = slot_create(c, r->cfg, r->params.data(), r->params.size(), e, &s);
+            if (!rc) { s->req = (long)r->ticket; r->slots.push_back(s); }
+        }
+        if (rc) {
+            for (Slot* s : r->slots) slot_destroy(c, s);
+            r->slots.clear();
+            r->state = 3;
+            r->status = rc;
+            r->t_done = now_s();
+            continue;
+        }
+        r->state = 1;
+        for (Slot* s : r->slots) c->active.push_back(s);
+    }
+    if (!c->active.empty()) {
+        // one epoch: cap every target at done + chunk
+        std::vector<Slot*> run;
+        for (Slot* s : c->active) {
+            s->target = std::min(s->su.cfg.steps, s->done + c->K);
+            run.push_back(s);
+        }
+        int rc = run_epochs(c, run);
+        if (rc) return rc;
+        // retire
+        std::vector<Slot*> still;
+        std::vector<tomo_request*> fin;
+        for (Slot* s : c->active) {
+            if (s->done >= s->su.cfg.steps) {
+                tomo_request* r = c->reqs[(uint64_t)s->req].get();
+                bool all = true;
+                for (Slot* q : r->slots) all = all && q->done >= q->su.cfg.steps;
+                if (all && std::find(fin.begin(), fin.end(), r) == fin.end()) fin.push_back(r);
+            }
+        }
+        for (Slot* s : c->active) {
+            tomo_request* r = c->reqs[(uint64_t)s->req].get();
+            if (std::find(fin.begin(), fin.end(), r) == fin.end()) still.push_back(s);
+        }
+        c->active.swap(still);
+        for (tomo_request* r : fin) finish_request(c, r);
+    }
+    if (n_active) *n_active = (int)c->active.size();
+    return TOMO_OK;
+}
+
+// ---------------------------------------------------------------- single simulation access
+struct tomo_sim {
+    tomo_ctx* ctx;
+    Slot* s;
+};
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* tomo_last_error(void) { return g_last_error.c_str(); }
+
+int tomo_preset(int preset_id, int grid, tomo_sim_config* out) { return preset(preset_id, grid, out); }
+
+int64_t tomo_farfield_len(const tomo_sim_config* cfg) {
+    return cfg ? (int64_t)cfg->n_exc * cfg->n_freq * cfg->n_angles : -1;
+}
+
+int32_t tomo_contour_len(const tomo_sim_config* cfg) { return cfg ? contour_len(*cfg) : -1; }
+
+int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx** out) {
+    if (!out) return fail(TOMO_E_INVALID, "null output");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(TOMO_E_INVALID, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<tomo_ctx> c(new tomo_ctx());
+    c->device = device;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
+    HIPCHK(hipEventCreateWithFlags(&c->staged, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->staged, c->stream));
+    int rc = ensure_table(c.get(), 256);
+    if (rc) return rc;
+    *out = c.release();
+    return TOMO_OK;
+}
+
+int tomo_ctx_destroy(tomo_ctx* c) {
+    if (!c) return TOMO_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (Slot* s : c->active) slot_destroy(c, s);
+    c->active.clear();
+    for (auto& kv : c->reqs)
+        for (Slot* s : kv.second->slots)
+            if (std::find(c->active.begin(), c->active.end(), s) == c->active.end()) slot_destroy(c, s);
+    if (c->d_sims) (void)hipFree(c->d_sims);
+    if (c->h_items) (void)hipHostFree(c->h_items);
+    if (c->d_items) (void)hipFree(c->d_items);
+    if (c->h_tiles) (void)hipHostFree(c->h_tiles);
+    if (c->d_tiles) (void)hipFree(c->d_tiles);
+    if (c->staged) (void)hipEventDestroy(c->staged);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TOMO_OK;
+}
+
+int tomo_set_schedule(tomo_ctx* c, int tsteps, int seg_rows, int waves, int chunk) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->active.empty()) return fail(TOMO_E_STATE, "schedule cannot change while requests are resident");
+    int T = tsteps ? tsteps : c->T, S = seg_rows ? seg_rows : c->S, W = waves ? waves : c->W,
+        K = chunk ? chunk : c->K;
+    if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
+    if (W < 1 || W > 8 || S < 4 || S > 4096 || K < T || K > 64 || K % T)
+        return fail(TOMO_E_INVALID, "bad schedule");
+    c->T = T; c->S = S; c->W = W; c->K = K;
+    return TOMO_OK;
+}
+
+int tomo_get_schedule(tomo_ctx* c, int* t, int* s, int* w, int* k) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    if (t) *t = c->T;
+    if (s) *s = c->S;
+    if (w) *w = c->W;
+    if (k) *k = c->K;
+    return TOMO_OK;
+}
+
+int tomo_eval(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n_params, const float* target,
+              size_t n_target, double* fitness_out, float* D_out, float* P_out) {
+    if (!c || !cfg) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    return eval_many(c, *cfg, params, 1, n_params, target, n_target, fitness_out, D_out, P_out, true);
+}
+
+int tomo_eval_batch(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n_vec, size_t dim,
+                    const float* target, size_t n_target, double* fitness_out, float* D_out) {
+    if (!c || !cfg) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    return eval_many(c, *cfg, params, n_vec, dim, target, n_target, fitness_out, D_out, nullptr, false);
+}
+
+int tomo_target_farfield(tomo_ctx* c, const tomo_sim_config* cfg, float* D_out) {
+    if (!c || !cfg || !D_out) return fail(TOMO_E_INVALID, "null argument");
+    if (cfg->mode != TOMO_MODE_PHYSICAL) return fail(TOMO_E_INVALID, "physical mode only");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::vector<float> t;
+    int rc = get_target(c, *cfg, t);
+    if (!rc) std::memcpy(D_out, t.data(), t.size() * 4);
+    return rc;
+}
+
+int64_t tomo_request_bytes(const tomo_sim_config* cfg) { return cfg ? request_bytes(*cfg, 32) : -1; }
+
+int tomo_submit(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n, uint64_t* ticket) {
+    if (!c || !cfg || !ticket || (!params && n)) return fail(TOMO_E_INVALID, "null argument");
+    if (cfg->mode != TOMO_MODE_PHYSICAL) return fail(TOMO_E_INVALID, "the request loop serves physical mode");
+    int rc = validate(*cfg);
+    if (rc) return rc;
+    if (n < 8) return fail(TOMO_E_SHORT, "physical mode needs at least 8 parameters");
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(TOMO_E_INVALID, "non-finite parameter");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::unique_ptr<tomo_request> r(new tomo_request());
+    r->ticket = c->next_ticket++;
+    r->cfg = *cfg;
+    r->params.assign(params, params + n);
+    r->t_submit = now_s();
+    r->bytes = request_bytes(*cfg, c->K);
+    *ticket = r->ticket;
+    c->queue.push_back(r->ticket);
+    c->reqs[r->ticket] = std::move(r);
+    return TOMO_OK;
+}
+
+int tomo_poll(tomo_ctx* c, uint64_t ticket, double* fitness_out, double* latency_s) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->reqs.find(ticket);
+    if (it == c->reqs.end()) return fail(TOMO_E_INVALID, "unknown ticket");
+    tomo_request* r = it->second.get();
+    if (r->state == 0 || r->state == 1) return TOMO_E_PENDING;
+    if (r->state == 4) return fail(TOMO_E_CANCELLED, "request cancelled");
+    if (fitness_out) *fitness_out = r->fitness;
+    if (latency_s) *latency_s = r->t_done - r->t_submit;
+    const int st = r->status;
+    if (st != TOMO_OK) g_last_error = "request failed";
+    c->reqs.erase(it);
+    return st;
+}
+
+int tomo_progress(tomo_ctx* c, int* n_active) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    return progress_once(c, n_active);
+}
+
+int tomo_wait(tomo_ctx* c, uint64_t ticket, double* fitness_out, double* latency_s) {
+    for (;;) {
+        int rc = tomo_poll(c, ticket, fitness_out, latency_s);
+        if (rc != TOMO_E_PENDING) return rc;
+        int na = 0;
+        rc = tomo_progress(c, &na);
+        if (rc) return rc;
+    }
+}
+
+int tomo_cancel(tomo_ctx* c, uint64_t ticket) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->reqs.find(ticket);
+    if (it == c->reqs.end()) return fail(TOMO_E_INVALID, "unknown ticket");
+    tomo_request* r = it->second.get();
+    if (r->state == 0) {
+        r->state = 4;
+        r->t_done = now_s();
+        return TOMO_OK;
+    }
+    if (r->state == 1) {
+        std::vector<Slot*> still;
+        for (Slot* s : c->active)
+            if (s->req != (long)r->ticket) still.push_back(s);
+        c->active.swap(still);
+        for (Slot* s : r->slots) slot_destroy(c, s);
+        r->slots.clear();
+        r->state = 4;
+        r->t_done = now_s();
+        return TOMO_OK;
+    }
+    return fail(TOMO_E_STATE, "request already finished");
+}
+
+int tomo_sim_create(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n, int exc,
+                    tomo_sim** out) {
+    if (!c || !cfg || !out) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    Slot* s = nullptr;
+    int rc = slot_create(c, *cfg, params, n, exc, &s);
+    if (rc) return rc;
+    *out = new tomo_sim{c, s};
+    return TOMO_OK;
+}
+
+int tomo_sim_destroy(tomo_sim* sim) {
+    if (!sim) return TOMO_OK;
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    slot_destroy(sim->ctx, sim->s);
+    delete sim;
+    return TOMO_OK;
+}
+
+int tomo_sim_step(tomo_sim* sim, int nsteps) {
+    if (!sim || nsteps < 0) return fail(TOMO_E_INVALID, "bad argument");
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    Slot* s = sim->s;
+    s->target = std::min(s->su.cfg.steps, s->done + nsteps);
+    std::vector<Slot*> one{s};
+    int rc = run_epochs(sim->ctx, one);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(sim->ctx->stream));
+    return TOMO_OK;
+}
+
+int tomo_sim_fields(tomo_sim* sim, float* ez, float* hx, float* hy, float* jp) {
+    if (!sim) return fail(TOMO_E_INVALID, "null sim");
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    Slot* s = sim->s;
+    const DevSim& d = s->d;
+    const int nx = d.nx, ny = d.ny, par = s->parity;
+    hipStream_t st = sim->ctx->stream;
+    if (ez) HIPCHK(hipMemcpy2DAsync(ez, (nx + 1) * 4, d.ez[par], d.pitch * 4, (nx + 1) * 4, ny + 1, hipMemcpyDeviceToHost, st));
+    if (hx) HIPCHK(hipMemcpy2DAsync(hx, nx * 4, d.hx[par], d.pitch * 4, nx * 4, ny, hipMemcpyDeviceToHost, st));
+    if (hy) HIPCHK(hipMemcpy2DAsync(hy, nx * 4, d.hy[par], d.pitch * 4, nx * 4, ny, hipMemcpyDeviceToHost, st));
+    if (jp) {
+        std::vector<float> cj((size_t)std::max(s->su.jp_len, 1));
+        HIPCHK(hipMemcpyAsync(cj.data(), d.jp[par], cj.size() * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::fill(jp, jp + (size_t)(nx + 1) * (ny + 1), 0.f);
+        for (int y = 0; y <= ny; ++y) {
+            const int lo = s->su.span[3 * y], hi = s->su.span[3 * y + 1], off = s->su.span[3 * y + 2];
+            for (int x = lo; x < hi; ++x) jp[(size_t)y * (nx + 1) + x] = cj[off + x - lo];
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return TOMO_OK;
+}
+
+int tomo_sim_materials(tomo_sim* sim, uint8_t* mat, float* coef, int* n_mat) {
+    if (!sim) return fail(TOMO_E_INVALID, "null sim");
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    Slot* s = sim->s;
+    const DevSim& d = s->d;
+    if (mat)
+        HIPCHK(hipMemcpy2D(mat, d.nx + 1, d.mat, d.pitch, d.nx + 1, d.ny + 1, hipMemcpyDeviceToHost));
+    if (coef) std::memcpy(coef, d.coef, sizeof(float) * d.nmat * TOMO_NCOEF);
+    if (n_mat) *n_mat = d.nmat;
+    return TOMO_OK;
+}
+
+int tomo_sim_dft(tomo_sim* sim, float* J, float* M) {
+    if (!sim) return fail(TOMO_E_INVALID, "null sim");
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    Slot* s = sim->s;
+    const DevSim& d = s->d;
+    const size_t n = (size_t)d.nfreq * d.P;
+    const bool f64 = (d.flags & TOMO_FLAG_DFT_FP64) != 0;
+    std::vector<float> a(4 * n), b(4 * n);
+    HIPCHK(hipMemcpy(a.data(), d.accJ, 16 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b.data(), d.accM, 16 * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        if (f64) {
+            const double* da = (const double*)a.data();
+            const double* db = (const double*)b.data();
+            if (J) { J[2 * i] = (float)da[2 * i]; J[2 * i + 1] = (float)da[2 * i + 1]; }
+            if (M) { M[2 * i] = (float)db[2 * i]; M[2 * i + 1] = (float)db[2 * i + 1]; }
+        } else {
+            if (J) { J[2 * i] = a[4 * i]; J[2 * i + 1] = a[4 * i + 1]; }
+            if (M) { M[2 * i] = b[4 * i]; M[2 * i + 1] = b[4 * i + 1]; }
+        }
+    }
+    return TOMO_OK;
+}
+
+int tomo_sim_farfield(tomo_sim* sim, float* D, float* P) {
+    if (!sim || !D) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(sim->ctx->mu);
+    Slot* s = sim->s;
+    const size_t L = (size_t)s->su.cfg.n_freq * s->su.cfg.n_angles;
+    float *dD = nullptr, *dP = nullptr;
+    HIPCHK(hipMalloc(&dD, L * 4));
+    HIPCHK(hipMalloc(&dP, L * 4));
+    std::vector<Slot*> one{s};
+    int rc = project(sim->ctx, one, {0}, dD, dP);
+    if (!rc) {
+        if (hipMemcpy(D, dD, L * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(TOMO_E_HIP, "copy");
+        if (!rc && P && hipMemcpy(P, dP, L * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(TOMO_E_HIP, "copy");
+    }
+    (void)hipFree(dD);
+    (void)hipFree(dP);
+    return rc;
+}
+
+}  // extern "C"
