@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2D TMz FDTD fitness evaluation on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One "step" is one fitness evaluation of the plain path on the 1024^2,
+dx = 0.25 mm, 8192-step workload (one antenna; one simulation per launch
+sequence), with every input already resident on the device.  `value` is
+Mcell-steps/s of the timed steps (cell-step = one Yee cell advanced one full
+E+H step).  The same JSON line carries:
+  * batched: one batch of the 512^2 x 4096-step workload, 64 parameter
+    vectors x 16 antennas = 1024 simulations advanced together (64 evals);
+  * roofline: the time-blocked update kernel's algorithmic bytes per launch
+    over its average launch time, timed with HIP events on the library stream;
+  * cpu_baseline: the oracle's FP32 restatement on one host core, on a bounded
+    sample of the same 1024^2 workload (a baseline, not the target);
+  * serving (with --serve): p50/p99 latency of a continuous-batching run.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import tomography_amd as t  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-steps/s & fitness evals/s, batched vs plain; p50/p99 eval latency"
+
+
+def seeded_vectors(n, dim, seed):
+    """Parameter vectors inside the physical bounds (tumour centre, radius, Debye values)."""
+    rng = np.random.default_rng(seed)
+    v = np.zeros((n, dim))
+    for b in range(0, dim, 8):
+        k = min(8, dim - b)
+        blk = np.stack([rng.uniform(-15, 15, n), rng.uniform(-15, 15, n), rng.uniform(2, 6, n),
+                        rng.uniform(10, 30, n), rng.uniform(10, 50, n), rng.uniform(8, 20, n),
+                        rng.uniform(0, 1, n), rng.uniform(0, 4, n)], axis=1)
+        v[:, b:b + k] = blk[:, :k]
+    return v
+
+
+def cell_steps(cfg):
+    return float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
+
+
+def _kernel_roofline(name, launches, ms, cells, alg_bytes):
+    if launches == 0 or ms <= 0:
+        return None
+    avg_s = ms / launches / 1e3
+    bpl = alg_bytes / launches
+    achieved = bpl / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bpl,
+            "alg_bytes_per_cell_step": round(alg_bytes / cells, 3) if cells else None,
+            "cell_steps_per_launch": cells / launches, "total_ms": round(ms, 2)}
+
+
+def roofline_from(stats):
+    """Roofline of the dominant update kernel (the other one is listed beside it)."""
+    inner = _kernel_roofline("k_tinner", stats["inner_launches"], stats["inner_ms"], stats["inner_cell_steps"],
+                             stats["inner_alg_bytes"])
+    edge = _kernel_roofline("k_tblock", stats["tblock_launches"], stats["tblock_ms"], stats["cell_steps"],
+                            stats["alg_bytes"])
+    ks = [k for k in (inner, edge) if k]
+    if not ks:
+        return None
+    dom = max(ks, key=lambda k: k["total_ms"])
+    dom = dict(dom)
+    dom["other_kernels"] = [k for k in ks if k["kernel"] != dom["kernel"]]
+    return dom
+
+
+def traffic_for(workload, schedule):
+    """Measured HBM bytes per update launch (profiles/k_tblock_traffic.json), if recorded."""
+    path = os.path.join(ROOT, "profiles", "k_tblock_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        rec = json.load(open(path)).get(workload)
+    except (OSError, ValueError):
+        return None
+    if rec and tuple(rec.get("schedule", ())) == tuple(schedule):
+        return rec.get("hbm_bytes_per_launch")
+    return None
+
+
+SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult")
+
+
+def run_plain(ctx, steps, warmup, tune=True):
+    cfg = t.preset(t.PRESET_CFG2)
+    if tune:
+        ctx.autotune(cfg, 1)
+    vecs = seeded_vectors(steps + warmup, 8, 20261202)
+    for i in range(warmup):
+        ctx.eval(cfg, vecs[i])  # the first call also builds and caches the target far field
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    fits = [ctx.eval(cfg, vecs[warmup + i]) for i in range(steps)]
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.profile(False)
+    assert all(np.isfinite(fits))
+    return cfg, dt, st
+
+
+def run_batched(ctx, tune=True):
+    cfg = t.preset(t.PRESET_CFG3)
+    if tune:
+        ctx.autotune(cfg, 128)
+    vecs = seeded_vectors(64, 8, 20261203)
+    ctx.target_farfield(cfg)  # warm-up: the 16-antenna target phantom
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    fits = ctx.eval_batch(cfg, vecs)
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.profile(False)
+    assert np.all(np.isfinite(fits))
+    return cfg, dt, st, len(vecs)
+
+
+def run_serve(ctx, n_req, rate, seed=20261205):
+    """Poisson arrivals of mixed requests (grids 256^2..1024^2, 1-4 antennas)."""
+    rng = np.random.default_rng(seed)
+    grids = np.exp(rng.uniform(np.log(256), np.log(1024), n_req)).astype(int) // 32 * 32
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        cfg = t.preset(t.PRESET_CFG5, int(grids[i])).copy(n_exc=int(rng.integers(1, 5)))
+        dim = int(rng.choice([8, 16, 24, 32]))
+        reqs.append((arrivals[i], cfg, seeded_vectors(1, dim, seed + i)[0]))
+    for g in sorted(set(int(x) for x in grids)):  # targets are part of the set-up, not the timed loop
+        for ne in range(1, 5):
+            ctx.target_farfield(t.preset(t.PRESET_CFG5, g).copy(n_exc=ne))
+    t0 = time.perf_counter()
+    pending, lat, done, i = {}, [], 0, 0
+    while done < n_req:
+        now = time.perf_counter() - t0
+        while i < n_req and reqs[i][0] <= now:
+            pending[ctx.submit(reqs[i][1], reqs[i][2])] = True
+            i += 1
+        if not pending:
+            time.sleep(max(0.0, min(0.01, reqs[i][0] - now)))
+            continue
+        ctx.progress()
+        for tk in list(pending):
+            ok, fit, l = ctx.poll(tk)
+            if ok:
+                lat.append(l)
+                del pending[tk]
+                done += 1
+    wall = time.perf_counter() - t0
+    cells = sum(cell_steps(r[1]) for r in reqs)
+    return {"requests": n_req, "rate_per_s": rate, "wall_s": round(wall, 3),
+            "evals_per_s": round(n_req / wall, 3), "Mcell_steps_per_s": round(cells / wall / 1e6, 1),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4)}
+
+
+def cpu_baseline(steps=2000):
+    """The oracle's FP32 restatement on one host core, first `steps` steps of
+    the 1024^2 plain workload (fields, CPML, Debye, running DFT)."""
+    from oracle.binding import Oracle
+    o = Oracle()
+    cfg = t.preset(t.PRESET_CFG2)
+    v = seeded_vectors(1, 8, 20261202)[0]
+    s = o.sim(cfg, v, exc=0, fp64=False)
+    t0 = time.perf_counter()
+    s.step(steps)
+    dt = time.perf_counter() - t0
+    mcs = cfg.nx * cfg.ny * steps / dt / 1e6
+    return {"value": round(mcs, 3), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32 (fdtd_oracle.cpp), 1024^2 cfg2 phantom, steps 0-{steps - 1} of 8192, 1 thread; "
+                      f"{dt:.1f} s", "evals_per_s": round(mcs * 1e6 / cell_steps(cfg), 6),
+            "host": platform_host()}
+
+
+def platform_host():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def hip_copy_calibration(mb):
+    """A plain device-to-device copy of known size (rocprof counter calibration)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = mb << 20
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study")
+    ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--schedule", type=str, default="",
+                    help="tsteps,seg_rows,edge_waves,chunk[,inner_waves,grid_mult]: fixed schedule, no autotuning")
+    ap.add_argument("--calib-copy-mb", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(simulations are independent: more GPUs would each run their own share).", file=sys.stderr)
+        return 2
+    ctx = t.Context()
+    tune = not args.schedule
+    if args.schedule:
+        ctx.set_schedule(*[int(x) for x in args.schedule.split(",")])
+    cfg, dt, st = run_plain(ctx, args.steps, args.warmup, tune)
+    sched = ctx.schedule()
+    cs = cell_steps(cfg) * args.steps
+    value = cs / dt / 1e6
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded phantoms)",
+           "config": {"workload": "plain: 1024^2 cells, dx 0.25 mm, 8192 steps, 1 antenna, 8 freq x 180 angles, "
+                                  "one simulation per launch sequence",
+                      "preset": "CFG2", "schedule": dict(zip(SCHED_KEYS, sched)),
+                      "schedule_source": "autotuned" if tune else "fixed"},
+           "evals_per_s": round(args.steps / dt, 3)}
+    rl = roofline_from(st)
+    if rl:
+        rl["traffic"] = traffic_for("plain_cfg2", sched)
+    out["roofline"] = rl
+    out["kernel_ms_per_step"] = {"k_tinner": round(st["inner_ms"] / args.steps, 3),
+                                 "k_tblock": round(st["tblock_ms"] / args.steps, 3),
+                                 "k_fold": round(st["fold_ms"] / args.steps, 3),
+                                 "k_project": round(st["project_ms"] / args.steps, 3)}
+    if not args.no_batched:
+        bcfg, bdt, bst, nvec = run_batched(ctx, tune)
+        bsched = ctx.schedule()
+        bcs = cell_steps(bcfg) * nvec
+        brl = roofline_from(bst)
+        if brl:
+            brl["traffic"] = traffic_for("batched_cfg3", bsched)
+        out["batched"] = {"workload": "512^2, dx 0.5 mm, 4096 steps; 64 vectors x 16 antennas = 1024 simulations, "
+                                      "16 freq x 360 angles (CFG3)",
+                          "Mcell_steps_per_s": round(bcs / bdt / 1e6, 1), "evals_per_s": round(nvec / bdt, 3),
+                          "s_per_batch": round(bdt, 3), "roofline": brl,
+                          "schedule": dict(zip(SCHED_KEYS, bsched)),
+                          "kernel_ms": {"k_tinner": round(bst["inner_ms"], 1), "k_tblock": round(bst["tblock_ms"], 1),
+                                        "k_fold": round(bst["fold_ms"], 1),
+                                        "k_project": round(bst["project_ms"], 1)}}
+        out["batched_vs_plain_cell_rate"] = round(out["batched"]["Mcell_steps_per_s"] / value, 3)
+    if args.serve:
+        out["serving"] = run_serve(ctx, args.serve, args.rate)
+    if args.calib_copy_mb:
+        hip_copy_calibration(args.calib_copy_mb)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"]["ratio_gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    ctx.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

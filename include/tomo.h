@@ -126,11 +126,45 @@ int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx **out);
 int tomo_ctx_destroy(tomo_ctx *ctx);
 const char *tomo_last_error(void);
 
-/* Kernel schedule knobs (the autotuner writes these; 0 = keep current).
- * tsteps: time steps fused per launch; seg_rows: rows per tile segment;
- * waves: waves per workgroup; chunk: running-DFT fold interval in steps. */
-int tomo_set_schedule(tomo_ctx *ctx, int tsteps, int seg_rows, int waves, int chunk);
-int tomo_get_schedule(tomo_ctx *ctx, int *tsteps, int *seg_rows, int *waves, int *chunk);
+/* Kernel schedule (the autotuner writes it; a zero field keeps the current
+ * value).  tsteps: time steps fused per launch (1, 2, 4, 6, 8); seg_rows:
+ * output rows per tile; edge_waves / inner_waves: waves per workgroup of the
+ * edge and interior update kernels (1-4); chunk: running-DFT fold interval in
+ * steps (a multiple of tsteps, <= 64); grid_mult: persistent workgroups per CU. */
+typedef struct tomo_schedule {
+    int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult;
+} tomo_schedule;
+int tomo_set_schedule(tomo_ctx *ctx, const tomo_schedule *s);
+int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
+
+/* Autotuner: time candidate schedules on n_sims resident simulations of
+ * `cfg` (coordinate descent over tsteps, seg_rows, edge/inner waves and
+ * grid_mult; the chunk is kept), install the fastest and return it with its
+ * measured update rate in cell-steps per second. */
+int tomo_autotune(tomo_ctx *ctx, const tomo_sim_config *cfg, int n_sims, tomo_schedule *best,
+                  double *cell_steps_per_s);
+
+/* Kernel timing: when enabled, every launch of the time-blocked update, the
+ * running-DFT fold and the projection is bracketed by HIP events on the
+ * library's stream.  alg_bytes counts the algorithmic HBM bytes of the update
+ * launches: cell-steps x (24 + 1 + 8 f_disp + 16 f_strip), where f_disp and
+ * f_strip are each simulation's fractions of Debye and CPML-strip cells. */
+typedef struct tomo_stats {
+    int64_t tblock_launches; /* general (edge-tile) update kernel */
+    double tblock_ms;
+    double cell_steps;
+    double alg_bytes;
+    int64_t inner_launches;  /* interior-tile update kernel */
+    double inner_ms;
+    double inner_cell_steps;
+    double inner_alg_bytes;
+    int64_t fold_launches;
+    double fold_ms;
+    int64_t project_launches;
+    double project_ms;
+} tomo_stats;
+int tomo_profile(tomo_ctx *ctx, int enable); /* (re)starts the counters */
+int tomo_get_stats(tomo_ctx *ctx, tomo_stats *out);
 
 /* ---- evaluation -------------------------------------------------------- */
 /*

@@ -21,11 +21,34 @@ _lib = None
 # Every symbol declared in include/tomo.h.
 EXPORTS = [
     "tomo_preset", "tomo_farfield_len", "tomo_contour_len", "tomo_ctx_create", "tomo_ctx_destroy",
-    "tomo_last_error", "tomo_set_schedule", "tomo_get_schedule", "tomo_eval", "tomo_eval_batch",
+    "tomo_last_error", "tomo_set_schedule", "tomo_get_schedule", "tomo_autotune", "tomo_eval", "tomo_eval_batch",
     "tomo_target_farfield", "tomo_submit", "tomo_poll", "tomo_wait", "tomo_cancel", "tomo_progress",
     "tomo_request_bytes", "tomo_sim_create", "tomo_sim_destroy", "tomo_sim_step", "tomo_sim_fields",
-    "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield",
+    "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_get_stats",
 ]
+
+
+class Schedule(_c.Structure):
+    """tomo_schedule (include/tomo.h)."""
+
+    _fields_ = [("tsteps", _c.c_int32), ("seg_rows", _c.c_int32), ("edge_waves", _c.c_int32),
+                ("inner_waves", _c.c_int32), ("chunk", _c.c_int32), ("grid_mult", _c.c_int32)]
+
+    def as_tuple(self):
+        return tuple(getattr(self, k) for k, _ in self._fields_)
+
+
+class Stats(_c.Structure):
+    """tomo_stats (include/tomo.h)."""
+
+    _fields_ = [("tblock_launches", _c.c_int64), ("tblock_ms", _c.c_double), ("cell_steps", _c.c_double),
+                ("alg_bytes", _c.c_double), ("inner_launches", _c.c_int64), ("inner_ms", _c.c_double),
+                ("inner_cell_steps", _c.c_double), ("inner_alg_bytes", _c.c_double),
+                ("fold_launches", _c.c_int64), ("fold_ms", _c.c_double),
+                ("project_launches", _c.c_int64), ("project_ms", _c.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TomoError(RuntimeError):
@@ -51,8 +74,9 @@ def lib():
     L.tomo_contour_len.restype = _c.c_int32
     L.tomo_ctx_create.argtypes = [_c.c_int, _c.c_uint64, _c.POINTER(vp)]
     L.tomo_ctx_destroy.argtypes = [vp]
-    L.tomo_set_schedule.argtypes = [vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]
-    L.tomo_get_schedule.argtypes = [vp, _ip, _ip, _ip, _ip]
+    L.tomo_set_schedule.argtypes = [vp, vp]
+    L.tomo_get_schedule.argtypes = [vp, vp]
+    L.tomo_autotune.argtypes = [vp, vp, _c.c_int, vp, _dp]
     L.tomo_eval.argtypes = [vp, vp, _dp, _c.c_size_t, _fp, _c.c_size_t, _dp, _fp, _fp]
     L.tomo_eval_batch.argtypes = [vp, vp, _dp, _c.c_size_t, _c.c_size_t, _fp, _c.c_size_t, _dp, _fp]
     L.tomo_target_farfield.argtypes = [vp, vp, _fp]
@@ -70,6 +94,8 @@ def lib():
     L.tomo_sim_materials.argtypes = [vp, _c.POINTER(_c.c_uint8), _fp, _ip]
     L.tomo_sim_dft.argtypes = [vp, _fp, _fp]
     L.tomo_sim_farfield.argtypes = [vp, _fp, _fp]
+    L.tomo_profile.argtypes = [vp, _c.c_int]
+    L.tomo_get_stats.argtypes = [vp, _c.c_void_p]
     _lib = L
     return L
 
@@ -93,12 +119,16 @@ class Context:
     """A device context: the owner of resident simulations and the request loop."""
 
     def __init__(self, device=0, hbm_budget=0):
+        import weakref
         h = _c.c_void_p()
         _check(lib().tomo_ctx_create(device, hbm_budget, _c.byref(h)))
         self.h = h
+        self._sims = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for s in list(self._sims):  # simulations die with their context
+                s.close()
             lib().tomo_ctx_destroy(self.h)
             self.h = None
 
@@ -111,13 +141,30 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def set_schedule(self, tsteps=0, seg_rows=0, waves=0, chunk=0):
-        _check(lib().tomo_set_schedule(self.h, tsteps, seg_rows, waves, chunk))
+    def set_schedule(self, tsteps=0, seg_rows=0, edge_waves=0, chunk=0, inner_waves=0, grid_mult=0):
+        s = Schedule(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult)
+        _check(lib().tomo_set_schedule(self.h, _c.byref(s)))
+
+    def autotune(self, cfg, n_sims):
+        """Pick and install the fastest schedule for n_sims resident simulations of cfg."""
+        s, r = Schedule(), _c.c_double()
+        _check(lib().tomo_autotune(self.h, _c.byref(cfg), int(n_sims), _c.byref(s), _c.byref(r)))
+        return s.as_tuple(), r.value
+
+    def profile(self, enable=True):
+        """Start (or stop) per-launch HIP-event timing; resets the counters."""
+        _check(lib().tomo_profile(self.h, int(enable)))
+
+    def stats(self):
+        st = Stats()
+        _check(lib().tomo_get_stats(self.h, _c.byref(st)))
+        return st.as_dict()
 
     def schedule(self):
-        v = [_c.c_int() for _ in range(4)]
-        _check(lib().tomo_get_schedule(self.h, *[_c.byref(x) for x in v]))
-        return tuple(x.value for x in v)
+        """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult)"""
+        s = Schedule()
+        _check(lib().tomo_get_schedule(self.h, _c.byref(s)))
+        return s.as_tuple()
 
     def eval(self, cfg, params, target=None, want_farfield=False):
         """One fitness evaluation (the FDTD_GPU() twin): returns fitness[, D, P]."""
@@ -188,14 +235,18 @@ class Sim:
 
     def __init__(self, ctx, cfg, params, exc=0):
         self.ctx, self.cfg = ctx, cfg
+        if not ctx.h:
+            raise TomoError(-8, "context is closed")
         p = np.ascontiguousarray(params, np.float64)
         h = _c.c_void_p()
         _check(lib().tomo_sim_create(ctx.h, _c.byref(cfg), _p(p, _dp), p.size, exc, _c.byref(h)))
         self.h = h
+        ctx._sims.add(self)
 
     def close(self):
         if getattr(self, "h", None):
-            lib().tomo_sim_destroy(self.h)
+            if self.ctx.h:  # a closed context already released the simulation's memory
+                lib().tomo_sim_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -10,9 +10,19 @@ namespace tomo {
 // Steps fused per launch that have a compiled kernel.
 bool tblock_supported(int T);
 
-// Time-blocked E/H update of every tile in `tiles` (T steps, one launch).
+// Time-blocked E/H update of every tile in `tiles` (T steps, one launch):
+// the general kernel (CPML, contour samples, edges, ref-compat physics) and
+// the interior kernel (physical mode, tiles away from every boundary).
 hipError_t launch_tblock(int T, int mode, const TileItem* tiles, int ntiles, const LaunchItem* items,
-                         const DevSim* sims, int ordinal, int seg_rows, int waves, int grid, hipStream_t st);
+                         const DevSim* sims, int ordinal, int waves, int grid, hipStream_t st);
+hipError_t launch_tedge(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                        int ordinal, int waves, int grid, hipStream_t st);
+hipError_t launch_trow(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                       int ordinal, int waves, int grid, hipStream_t st);
+hipError_t launch_tcol(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                       int ordinal, int waves, int grid, hipStream_t st);
+hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                         int ordinal, int waves, int grid, hipStream_t st);
 
 // Fold one chunk of surface samples into the running DFT (Kahan FP32 or FP64).
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,

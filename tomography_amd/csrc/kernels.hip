@@ -27,6 +27,23 @@ namespace {
 
 constexpr int kWave = 64;
 
+// Global-address-space views: loads and stores through these compile to
+// global_* instructions (not flat_*), so LDS waits never wait on HBM.
+#define GAS __attribute__((address_space(1)))
+typedef GAS float gfloat;
+typedef GAS const float gcfloat;
+typedef GAS const uint8_t gcu8;
+typedef GAS const int gcint;
+
+// Lane shifts on the VALU (DPP wave_shl:1 / wave_shr:1): next(v)[i] = v[i+1],
+// prev(v)[i] = v[i-1].  The edge lane keeps a stale value; it is a ghost lane.
+__device__ __forceinline__ float lane_next(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+
 struct Cell {
     float ez, hx, hy, jp, pex, phx, pey, phy;
     int mat;
@@ -39,63 +56,101 @@ __device__ __forceinline__ Cell zero_cell() {
     return c;
 }
 
-// Scalar (wave-uniform) view of a DevSim, held in SGPRs.
+// Wave-uniform view of one simulation for one launch (kept in SGPRs).
 struct SimView {
-    float *ez, *hx, *hy, *jp, *pex, *phx, *pey, *phy;   // buffers read (parity par)
-    float *oez, *ohx, *ohy, *ojp, *opex, *ophx, *opey, *ophy;  // buffers written
-    const uint8_t* mat;
-    const int* span;
-    float* samp;
-    const float* srcw;
+    gfloat* f;     // field planes: ez0 ez1 hx0 hx1 hy0 hy1, stride ps
+    gfloat* jp;    // jp0 jp1, stride js
+    gfloat* xq;    // pex0 pex1 phx0 phx1, stride xs
+    gfloat* yq;    // pey0 pey1 phy0 phy1, stride ys
+    gcu8* mat;
+    gcint* span;
+    gfloat* samp;
+    gcfloat* srcw;
+    int ps, js, xs, ys, par;
     int nx, ny, pitch, nc, D, P, L1, L2, L3, cx_hi, cy_lo, sx, sy, flags, K;
     float chx, chy, chx_d, chy_d, dxf, dyf;
     float inc_dt, inc_c, inc_ax, inc_ay, inc_itau;
     int inc_abs;
 };
 
-__device__ __forceinline__ void load_cell(const SimView& S, const float* __restrict__ lcoef, int r, int x,
-                                          bool xin, Cell& c) {
-    c = zero_cell();
-    if (r < 0 || r > S.ny || !xin) return;
-    const size_t base = (size_t)r * S.pitch + x;
-    c.ez = S.ez[base];
-    c.mat = S.mat[base];
-    if (x < S.nx && r < S.ny) {
-        c.hx = S.hx[base];
-        c.hy = S.hy[base];
-    }
-    const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (x >= lo && x < hi && lcoef[c.mat * TOMO_NCOEF + 7] != 0.f) c.jp = S.jp[S.span[3 * r + 2] + x - lo];
-    const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
-    if (x >= 1 && x <= nc) c.pex = S.pex[r * w + x - 1];
-    else if (x >= nx - nc && x <= nx - 1) c.pex = S.pex[r * w + nc + x - (nx - nc)];
-    if (x < nc) c.phx = S.phx[r * w + x];
-    else if (x >= nx - nc && x < nx) c.phx = S.phx[r * w + nc + x - (nx - nc)];
-    if (r >= 1 && r <= nc) c.pey = S.pey[(size_t)(r - 1) * S.pitch + x];
-    else if (r >= ny - nc && r <= ny - 1) c.pey = S.pey[(size_t)(nc + r - (ny - nc)) * S.pitch + x];
-    if (r < nc) c.phy = S.phy[(size_t)r * S.pitch + x];
-    else if (r >= ny - nc && r < ny) c.phy = S.phy[(size_t)(nc + r - (ny - nc)) * S.pitch + x];
+// Wave-uniform facts about a band of 64 columns [x0, x0 + 63].
+struct WaveFlags {
+    bool xs;     // holds CPML x-strip cells
+    bool ccol;   // holds an NTFF contour column or its neighbour
+    bool src;    // holds the source column
+};
+
+__device__ __forceinline__ bool row_ystrip_e(const SimView& S, int r) {
+    return (r >= 1 && r <= S.nc) || (r >= S.ny - S.nc && r <= S.ny - 1);
+}
+__device__ __forceinline__ bool row_ystrip_h(const SimView& S, int r) {
+    return (r >= 0 && r < S.nc) || (r >= S.ny - S.nc && r < S.ny);
 }
 
-__device__ __forceinline__ void store_cell(const SimView& S, const float* __restrict__ lcoef, int r, int x,
-                                           const Cell& c) {
-    const size_t base = (size_t)r * S.pitch + x;
-    S.oez[base] = c.ez;
+__device__ __forceinline__ void load_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x, bool xin,
+                                          const WaveFlags& wf, Cell& c) {
+    c = zero_cell();
+    if (r < 0 || r > S.ny) return;  // wave-uniform
+    if (!xin) return;
+    const int base = r * S.pitch + x;
+    const int in_f = S.par * S.ps;
+    c.ez = S.f[in_f + base];
+    c.mat = S.mat[base];
     if (x < S.nx && r < S.ny) {
-        S.ohx[base] = c.hx;
-        S.ohy[base] = c.hy;
+        c.hx = S.f[in_f + 2 * S.ps + base];
+        c.hy = S.f[in_f + 4 * S.ps + base];
     }
     const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (x >= lo && x < hi && lcoef[c.mat * TOMO_NCOEF + 7] != 0.f) S.ojp[S.span[3 * r + 2] + x - lo] = c.jp;
+    if (hi > lo && x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f)
+        c.jp = S.jp[S.par * S.js + S.span[3 * r + 2] + x - lo];
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
-    if (x >= 1 && x <= nc) S.opex[r * w + x - 1] = c.pex;
-    else if (x >= nx - nc && x <= nx - 1) S.opex[r * w + nc + x - (nx - nc)] = c.pex;
-    if (x < nc) S.ophx[r * w + x] = c.phx;
-    else if (x >= nx - nc && x < nx) S.ophx[r * w + nc + x - (nx - nc)] = c.phx;
-    if (r >= 1 && r <= nc) S.opey[(size_t)(r - 1) * S.pitch + x] = c.pey;
-    else if (r >= ny - nc && r <= ny - 1) S.opey[(size_t)(nc + r - (ny - nc)) * S.pitch + x] = c.pey;
-    if (r < nc) S.ophy[(size_t)r * S.pitch + x] = c.phy;
-    else if (r >= ny - nc && r < ny) S.ophy[(size_t)(nc + r - (ny - nc)) * S.pitch + x] = c.phy;
+    if (wf.xs) {
+        gfloat* pe = S.xq + S.par * S.xs;
+        gfloat* ph = S.xq + (2 + S.par) * S.xs;
+        if (x >= 1 && x <= nc) c.pex = pe[r * w + x - 1];
+        else if (x >= nx - nc && x <= nx - 1) c.pex = pe[r * w + nc + x - (nx - nc)];
+        if (x < nc) c.phx = ph[r * w + x];
+        else if (x >= nx - nc && x < nx) c.phx = ph[r * w + nc + x - (nx - nc)];
+    }
+    if (row_ystrip_e(S, r)) {
+        const int j = r <= nc ? r - 1 : nc + r - (ny - nc);
+        c.pey = S.yq[S.par * S.ys + j * S.pitch + x];
+    }
+    if (row_ystrip_h(S, r)) {
+        const int j = r < nc ? r : nc + r - (ny - nc);
+        c.phy = S.yq[(2 + S.par) * S.ys + j * S.pitch + x];
+    }
+}
+
+__device__ __forceinline__ void store_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x,
+                                           const WaveFlags& wf, const Cell& c) {
+    const int base = r * S.pitch + x;
+    const int out_f = (S.par ^ 1) * S.ps;
+    S.f[out_f + base] = c.ez;
+    if (x < S.nx && r < S.ny) {
+        S.f[out_f + 2 * S.ps + base] = c.hx;
+        S.f[out_f + 4 * S.ps + base] = c.hy;
+    }
+    const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
+    if (hi > lo && x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f)
+        S.jp[(S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo] = c.jp;
+    const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
+    if (wf.xs) {
+        gfloat* pe = S.xq + (S.par ^ 1) * S.xs;
+        gfloat* ph = S.xq + (2 + (S.par ^ 1)) * S.xs;
+        if (x >= 1 && x <= nc) pe[r * w + x - 1] = c.pex;
+        else if (x >= nx - nc && x <= nx - 1) pe[r * w + nc + x - (nx - nc)] = c.pex;
+        if (x < nc) ph[r * w + x] = c.phx;
+        else if (x >= nx - nc && x < nx) ph[r * w + nc + x - (nx - nc)] = c.phx;
+    }
+    if (row_ystrip_e(S, r)) {
+        const int j = r <= nc ? r - 1 : nc + r - (ny - nc);
+        S.yq[(S.par ^ 1) * S.ys + j * S.pitch + x] = c.pey;
+    }
+    if (row_ystrip_h(S, r)) {
+        const int j = r < nc ? r : nc + r - (ny - nc);
+        S.yq[(2 + (S.par ^ 1)) * S.ys + j * S.pitch + x] = c.phy;
+    }
 }
 
 // LDS CPML table layout: [0] ke [1] km [2] ike [3] ikm [4] ae [5] be [6] am [7] bm, each TOMO_MAX_NC.
@@ -105,39 +160,43 @@ __device__ __forceinline__ void store_cell(const SimView& S, const float* __rest
 // H^{n-1/2}); ez_up = Ez^n(x, y+1); hx_prev = Hx^{n+1/2}(x, y-1) on entry and
 // Hx^{n+1/2}(x, y) on exit.  On exit c holds row y at level n+1.
 template <int MODE>
-__device__ __forceinline__ void stage_update(const SimView& S, const float* __restrict__ lcoef,
-                                             const float* __restrict__ lcp, Cell& c, float ez_up, float& hx_prev,
-                                             int x, int y, int n) {
+__device__ __forceinline__ void stage_update(const SimView& S, const float4* __restrict__ lc4,
+                                             const float* __restrict__ lcp, const WaveFlags& wf, Cell& c,
+                                             float ez_up, float& hx_prev, int x, int y, int n) {
     const int nx = S.nx, ny = S.ny, nc = S.nc;
     // ---------------- H half-step (H_field_update, FDTD_GPU.cu:298-355)
-    const float ez_r = __shfl_down(c.ez, 1);
+    const float ez_r = lane_next(c.ez);
     const float dEx = ez_r - c.ez, dEy = ez_up - c.ez;
     float hy, hx;
     // H nodes exist for 0 <= x < nx, 0 <= y < ny; the CPML branches run only there
-    const bool hx_in = (x >= 0 && x < nx), hy_in = (y >= 0 && y < ny);
+    const bool hx_in = (x >= 0 && x < nx);
+    const bool hy_in = (y >= 0 && y < ny);        // wave-uniform
+    const bool ys_h = hy_in && (y < nc || y >= ny - nc);  // wave-uniform
     if (MODE == TOMO_MODE_REF_COMPAT) {
         const float chez = S.chx;
         hy = c.hy + chez * dEx;
         hx = c.hx - chez * dEy;
-        if (hx_in && x < nc) {
-            const int i = nc - 1 - x;
-            hy = c.hy + chez * dEx / CP(1, i);
-            c.phx = CP(7, i) * c.phx + CP(6, i) * dEx;
-            hy = hy + S.chy_d * c.phx;
+        if (wf.xs) {
+            if (hx_in && x < nc) {
+                const int i = nc - 1 - x;
+                hy = c.hy + chez * dEx / CP(1, i);
+                c.phx = CP(7, i) * c.phx + CP(6, i) * dEx;
+                hy = hy + S.chy_d * c.phx;
+            }
+            if (hx_in && x >= nx - nc) {
+                const int i = x - nx + nc;
+                hy = c.hy + chez * dEx / CP(1, i);
+                c.phx = CP(7, i) * c.phx + CP(6, i) * dEx;
+                hy = hy + S.chy_d * c.phx;
+            }
         }
-        if (hx_in && x >= nx - nc) {
-            const int i = x - nx + nc;
-            hy = c.hy + chez * dEx / CP(1, i);
-            c.phx = CP(7, i) * c.phx + CP(6, i) * dEx;
-            hy = hy + S.chy_d * c.phx;
-        }
-        if (hy_in && y < nc) {
+        if (ys_h && y < nc) {
             const int i = nc - 1 - y;
             hx = c.hx - chez * dEy / CP(1, i);
             c.phy = CP(7, i) * c.phy + CP(6, i) * dEy;
             hx = hx - S.chx_d * c.phy;
         }
-        if (hy_in && y >= ny - nc) {
+        if (ys_h && y >= ny - nc) {
             const int i = y - ny + nc;
             hx = c.hx - chez * dEy / CP(1, i);
             c.phy = CP(7, i) * c.phy + CP(6, i) * dEy;
@@ -149,12 +208,12 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float* __re
         }
     } else {
         float ty = dEx, tx = dEy;
-        if (hx_in && (x < nc || x >= nx - nc)) {
+        if (wf.xs && hx_in && (x < nc || x >= nx - nc)) {
             const int i = x < nc ? nc - 1 - x : x - nx + nc;
             c.phx = CP(7, i) * c.phx + CP(6, i) * dEx;
             ty = dEx * CP(3, i) + S.dxf * c.phx;
         }
-        if (hy_in && (y < nc || y >= ny - nc)) {
+        if (ys_h) {
             const int i = y < nc ? nc - 1 - y : y - ny + nc;
             c.phy = CP(7, i) * c.phy + CP(6, i) * dEy;
             tx = dEy * CP(3, i) + S.dyf * c.phy;
@@ -167,20 +226,25 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float* __re
         hy = 0.f;
     }
     // ---------------- E half-step (E_field_update, FDTD_GPU.cu:358-434)
-    const float hy_l = __shfl_up(hy, 1);
+    const float hy_l = lane_prev(hy);
     const float dHy = hy - hy_l, dHx = hx - hx_prev;
     hx_prev = hx;
     c.hx = hx;
     c.hy = hy;
-    if (x <= 0 || x >= nx || y <= 0 || y >= ny) {
+    if (y <= 0 || y >= ny) {  // wave-uniform: PEC rows and rows outside the grid
         c.ez = 0.f;
         return;
     }
-    const float* q = lcoef + c.mat * TOMO_NCOEF;
+    if (x <= 0 || x >= nx) {
+        c.ez = 0.f;
+        return;
+    }
+    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];  // Ceze Cezh Cezeic Cezeip | Cezjp Kp BpDt disp
     const float ez = c.ez;
+    const bool ys_e = (y <= nc || y >= ny - nc);  // wave-uniform
     if (MODE == TOMO_MODE_REF_COMPAT) {
         float f1 = 0.f, f0 = 0.f;
-        if (q[2] != 0.f || q[3] != 0.f || q[6] != 0.f) {
+        if (qa.z != 0.f || qa.w != 0.f || qb.z != 0.f) {
             const float X = S.inc_abs ? (float)x : (float)(x - nx / 2);
             const float Y = S.inc_abs ? (float)y : (float)(y - ny / 2);
             const float sp = S.inc_c - S.inc_ay * Y - S.inc_ax * X;
@@ -189,49 +253,51 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float* __re
             f1 = expf(-a1 * a1);
             f0 = expf(-a0 * a0);
         }
-        float buf = q[0] * ez + q[1] * dHy - q[1] * dHx + q[2] * f1 + q[3] * f0 - q[4] * c.jp;
-        if (!(S.flags & TOMO_FLAG_CPU_COEFS)) c.jp = q[5] * c.jp + q[6] * (buf - ez + f1 - f0);
-        if (x <= nc) {
-            const int i = nc - x;
-            buf = q[0] * ez + q[1] * dHy / CP(0, i) - q[1] * dHx / CP(0, i);
-            c.pex = CP(5, i) * c.pex + CP(4, i) * dHy;
-            buf = buf + q[1] * S.dxf * c.pex;
+        float buf = qa.x * ez + qa.y * dHy - qa.y * dHx + qa.z * f1 + qa.w * f0 - qb.x * c.jp;
+        if (!(S.flags & TOMO_FLAG_CPU_COEFS)) c.jp = qb.y * c.jp + qb.z * (buf - ez + f1 - f0);
+        if (wf.xs) {
+            if (x <= nc) {
+                const int i = nc - x;
+                buf = qa.x * ez + qa.y * dHy / CP(0, i) - qa.y * dHx / CP(0, i);
+                c.pex = CP(5, i) * c.pex + CP(4, i) * dHy;
+                buf = buf + qa.y * S.dxf * c.pex;
+            }
+            if (x >= nx - nc) {
+                const int i = x - nx + nc;
+                buf = qa.x * ez + qa.y * dHy / CP(0, i) - qa.y * dHx / CP(0, i);
+                c.pex = CP(5, i) * c.pex + CP(4, i) * dHy;
+                buf = buf + qa.y * S.dxf * c.pex;
+            }
         }
-        if (x >= nx - nc) {
-            const int i = x - nx + nc;
-            buf = q[0] * ez + q[1] * dHy / CP(0, i) - q[1] * dHx / CP(0, i);
-            c.pex = CP(5, i) * c.pex + CP(4, i) * dHy;
-            buf = buf + q[1] * S.dxf * c.pex;
-        }
-        if (y <= nc) {
+        if (ys_e && y <= nc) {
             const int i = nc - y;
-            buf = q[0] * ez + q[1] * dHy / CP(0, i) - q[1] * dHx / CP(0, i);
+            buf = qa.x * ez + qa.y * dHy / CP(0, i) - qa.y * dHx / CP(0, i);
             c.pey = CP(5, i) * c.pey + CP(4, i) * dHx;
-            buf = buf - q[1] * S.dyf * c.pey;
+            buf = buf - qa.y * S.dyf * c.pey;
         }
-        if (y >= ny - nc) {
+        if (ys_e && y >= ny - nc) {
             const int i = y - ny + nc;
-            buf = q[0] * ez + q[1] * dHy / CP(0, i) - q[1] * dHx / CP(0, i);
+            buf = qa.x * ez + qa.y * dHy / CP(0, i) - qa.y * dHx / CP(0, i);
             c.pey = CP(5, i) * c.pey + CP(4, i) * dHx;
-            buf = buf - q[1] * S.dyf * c.pey;
+            buf = buf - qa.y * S.dyf * c.pey;
         }
         if (!(S.flags & TOMO_FLAG_NO_NAN_CLAMP) && __builtin_isnan(buf)) buf = 0.f;
         c.ez = buf;
     } else {
         float cy = dHy, cx = dHx;
-        if (x <= nc || x >= nx - nc) {
+        if (wf.xs && (x <= nc || x >= nx - nc)) {
             const int i = x <= nc ? nc - x : x - nx + nc;
             c.pex = CP(5, i) * c.pex + CP(4, i) * dHy;
             cy = dHy * CP(2, i) + S.dxf * c.pex;
         }
-        if (y <= nc || y >= ny - nc) {
+        if (ys_e) {
             const int i = y <= nc ? nc - y : y - ny + nc;
             c.pey = CP(5, i) * c.pey + CP(4, i) * dHx;
             cx = dHx * CP(2, i) + S.dyf * c.pey;
         }
-        float en = q[0] * ez + q[1] * cy - q[1] * cx - q[4] * c.jp;
-        if (x == S.sx && y == S.sy) en += S.srcw[n];
-        c.jp = q[5] * c.jp + q[6] * (en - ez);
+        float en = qa.x * ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
+        if (wf.src && y == S.sy && x == S.sx) en += S.srcw[n];
+        c.jp = qb.y * c.jp + qb.z * (en - ez);
         c.ez = en;
     }
 }
@@ -239,25 +305,31 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float* __re
 // Surface samples of the NTFF contour (contour map of FDTD_common.cxx:4-46):
 // plane 0 = H on the face cell, 1 = Ez on the face cell, 2 = Ez on the cell
 // across the face.  The running DFT averages planes 1 and 2.
-__device__ __forceinline__ void write_samples(const SimView& S, const Cell& c, int x, int y, int k) {
+__device__ __forceinline__ void write_samples(const SimView& S, const WaveFlags& wf, const Cell& c, int x, int y,
+                                              int k) {
     const int D = S.D, nx = S.nx, ny = S.ny, P = S.P;
-    float* h_ = S.samp + (size_t)k * P;
-    float* e0 = S.samp + (size_t)(S.K + k) * P;
-    float* e1 = S.samp + (size_t)(2 * S.K + k) * P;
-    const bool xr = (x >= D + 1 && x <= S.cx_hi);
-    if (y == D && xr) {
-        const int p = x - D - 1;
-        h_[p] = c.hx;
-        e0[p] = c.ez;
+    const bool yrow = (y == D || y == D + 1 || y == ny - D - 1 || y == ny - D);  // wave-uniform
+    const bool xrange = (y >= S.cy_lo && y <= ny - D - 1);                      // wave-uniform
+    if (!yrow && !(wf.ccol && xrange)) return;
+    gfloat* h_ = S.samp + k * P;
+    gfloat* e0 = S.samp + (S.K + k) * P;
+    gfloat* e1 = S.samp + (2 * S.K + k) * P;
+    if (yrow) {
+        const bool xr = (x >= D + 1 && x <= S.cx_hi);
+        if (y == D && xr) {
+            const int p = x - D - 1;
+            h_[p] = c.hx;
+            e0[p] = c.ez;
+        }
+        if (y == ny - D - 1 && xr) {
+            const int p = S.L2 + (S.cx_hi - x);
+            h_[p] = c.hx;
+            e0[p] = c.ez;
+        }
+        if (y == D + 1 && xr) e1[x - D - 1] = c.ez;
+        if (y == ny - D && xr) e1[S.L2 + (S.cx_hi - x)] = c.ez;
     }
-    if (y == ny - D - 1 && xr) {
-        const int p = S.L2 + (S.cx_hi - x);
-        h_[p] = c.hx;
-        e0[p] = c.ez;
-    }
-    if (y == D + 1 && xr) e1[x - D - 1] = c.ez;
-    if (y == ny - D && xr) e1[S.L2 + (S.cx_hi - x)] = c.ez;
-    if (y >= S.cy_lo && y <= ny - D - 1) {
+    if (wf.ccol && xrange) {
         if (x == nx - D - 1) {
             const int p = S.L1 + y - S.cy_lo;
             h_[p] = c.hy;
@@ -276,8 +348,8 @@ __device__ __forceinline__ void write_samples(const SimView& S, const Cell& c, i
 template <int T, int MODE>
 __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ tiles, int ntiles,
                                                 const LaunchItem* __restrict__ items,
-                                                const DevSim* __restrict__ sims, int ordinal, int seg_rows) {
-    __shared__ float lcoef[TOMO_MAX_MAT * TOMO_NCOEF];
+                                                const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     __shared__ float lcp[8 * TOMO_MAX_NC];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
@@ -289,33 +361,39 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
         if (nst <= 0) continue;  // uniform over the workgroup
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcoef[i] = Sp->coef[i];
+        {
+            float* lcf = (float*)lc4;
+            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        }
         for (int i = threadIdx.x; i < Sp->nc; i += blockDim.x) {
             CP(0, i) = Sp->ke[i]; CP(1, i) = Sp->km[i]; CP(2, i) = Sp->ike[i]; CP(3, i) = Sp->ikm[i];
             CP(4, i) = Sp->ae[i]; CP(5, i) = Sp->be[i]; CP(6, i) = Sp->am[i]; CP(7, i) = Sp->bm[i];
         }
         __syncthreads();
-        const int par = it.parity0 ^ (ordinal & 1);
         SimView S;
-        S.ez = Sp->ez[par]; S.hx = Sp->hx[par]; S.hy = Sp->hy[par]; S.jp = Sp->jp[par];
-        S.pex = Sp->pex[par]; S.phx = Sp->phx[par]; S.pey = Sp->pey[par]; S.phy = Sp->phy[par];
-        S.oez = Sp->ez[par ^ 1]; S.ohx = Sp->hx[par ^ 1]; S.ohy = Sp->hy[par ^ 1]; S.ojp = Sp->jp[par ^ 1];
-        S.opex = Sp->pex[par ^ 1]; S.ophx = Sp->phx[par ^ 1]; S.opey = Sp->pey[par ^ 1]; S.ophy = Sp->phy[par ^ 1];
-        S.mat = Sp->mat; S.span = Sp->span; S.samp = Sp->samp; S.srcw = Sp->srcw;
+        S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
+        S.mat = (gcu8*)Sp->mat; S.span = (gcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (gcfloat*)Sp->srcw;
+        S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
+        S.par = it.parity0 ^ (ordinal & 1);
         S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
-        S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo; S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags;
-        S.K = Sp->chunk;
+        S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
+        S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
         S.chx = Sp->chx; S.chy = Sp->chy; S.chx_d = Sp->chx_d; S.chy_d = Sp->chy_d; S.dxf = Sp->dxf;
         S.dyf = Sp->dyf; S.inc_dt = Sp->inc_dt; S.inc_c = Sp->inc_c; S.inc_ax = Sp->inc_ax;
         S.inc_ay = Sp->inc_ay; S.inc_itau = Sp->inc_itau; S.inc_abs = Sp->inc_abs_origin;
-        const int nbands = (S.nx + 1 + V - 1) / V;
-        const int band = tile.band0 + wv;
-        if (band < nbands) {
-            const int x = band * V - T + lane;
+        if (wv < tile.nband) {
+            const int bx = tile.x0 + wv * V;
+            const int x0 = bx - T;
+            const int x = x0 + lane;
             const bool xin = (x >= 0 && x <= S.nx);
-            const bool xown = xin && lane >= T && lane < kWave - T;
-            const int y0 = tile.seg * seg_rows;
-            const int y1 = min(y0 + seg_rows, S.ny + 1);
+            const bool xown = xin && lane >= T && x < min(bx + V, tile.xe);
+            WaveFlags wf;
+            wf.xs = (x0 <= S.nc) || (x0 + kWave - 1 >= S.nx - S.nc);
+            wf.ccol = (x0 <= S.D + 1 && x0 + kWave - 1 >= S.D) ||
+                      (x0 <= S.nx - S.D && x0 + kWave - 1 >= S.nx - S.D - 1);
+            wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
+            const int y0 = tile.y0;
+            const int y1 = tile.y1;
             const int step0 = it.step0 + ordinal * T;
             const int k0 = ordinal * T;
             Cell cur[T];
@@ -326,28 +404,603 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
                 hxp[t] = 0.f;
             }
             Cell nxt;
-            load_cell(S, lcoef, y0 - T, x, xin, nxt);
+            load_cell(S, lc4, y0 - T, x, xin, wf, nxt);
             for (int r = y0 - T; r < y1 + T; ++r) {
                 Cell in = nxt;
-                if (r + 1 < y1 + T) load_cell(S, lcoef, r + 1, x, xin, nxt);
+                if (r + 1 < y1 + T) load_cell(S, lc4, r + 1, x, xin, wf, nxt);
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
                     const int y = r - 1 - t;
                     Cell out = cur[t];
                     if (t < nst) {
-                        stage_update<MODE>(S, lcoef, lcp, out, in.ez, hxp[t], x, y, step0 + t);
-                        if (xown && y >= y0 && y < y1) write_samples(S, out, x, y, k0 + t);
+                        stage_update<MODE>(S, lc4, lcp, wf, out, in.ez, hxp[t], x, y, step0 + t);
+                        if (y >= y0 && y < y1 && xown) write_samples(S, wf, out, x, y, k0 + t);
                     }
                     cur[t] = in;
                     in = out;
                 }
                 const int yo = r - T;
-                if (xown && yo >= y0 && yo < y1) store_cell(S, lcoef, yo, x, in);
+                if (yo >= y0 && yo < y1 && xown) store_cell(S, lc4, yo, x, wf, in);
             }
         }
     }
 }
+// Edge tiles of physical-mode simulations: the CPML recursions are applied
+// to every cell through coefficient tables whose identity entry (1/kappa = 1,
+// a = b = 0) covers the cells outside the strips, so strip cells and ordinary
+// cells take the same instruction stream; PEC and out-of-grid nodes are
+// lane/row selects.  x-strip coefficients are lane constants held in
+// registers, y-strip coefficients are wave-uniform per row.  Ordinary cells
+// compute exactly what k_tinner computes (multiplications by 1 and additions
+// of 0 are exact).
+template <int T>
+__global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tiles, int ntiles,
+                                               const LaunchItem* __restrict__ items,
+                                               const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcp[8 * TOMO_MAX_NC];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    constexpr int V = kWave - 2 * T;
+    constexpr int ID = TOMO_MAX_NC - 1;  // identity entry of the CPML tables
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;
+        const DevSim* Sp = sims + it.sim;
+        __syncthreads();
+        {
+            float* lcf = (float*)lc4;
+            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        }
+        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+            const bool id = (i >= Sp->nc);
+            CP(2, i) = id ? 1.f : Sp->ike[i];
+            CP(3, i) = id ? 1.f : Sp->ikm[i];
+            CP(4, i) = id ? 0.f : Sp->ae[i];
+            CP(5, i) = id ? 0.f : Sp->be[i];
+            CP(6, i) = id ? 0.f : Sp->am[i];
+            CP(7, i) = id ? 0.f : Sp->bm[i];
+        }
+        __syncthreads();
+        if (wv >= tile.nband) continue;  // after the last barrier of this tile
+        SimView S;
+        S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
+        S.mat = (gcu8*)Sp->mat; S.span = (gcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (gcfloat*)Sp->srcw;
+        S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
+        S.par = it.parity0 ^ (ordinal & 1);
+        S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
+        S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
+        S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
+        S.chx = Sp->chx; S.chy = Sp->chy; S.dxf = Sp->dxf; S.dyf = Sp->dyf;
+        const int nx = S.nx, ny = S.ny, nc = S.nc;
+        const int bx = tile.x0 + wv * V;
+        const int x0 = bx - T;
+        const int x = x0 + lane;
+        const bool xin = (x >= 0 && x <= nx);
+        const bool xown = xin && lane >= T && x < min(bx + V, tile.xe);
+        WaveFlags wf;
+        wf.xs = (x0 <= nc) || (x0 + kWave - 1 >= nx - nc);
+        wf.ccol = (x0 <= S.D + 1 && x0 + kWave - 1 >= S.D) || (x0 <= nx - S.D && x0 + kWave - 1 >= nx - S.D - 1);
+        wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
+        // lane constants: x-strip coefficients and existence masks
+        const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+        const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+        const float kxe = CP(2, ixe), axe = CP(4, ixe), bxe = CP(5, ixe);
+        const float kxh = CP(3, ixh), axh = CP(6, ixh), bxh = CP(7, ixh);
+        const bool hx_ok = (x >= 0 && x < nx), ex_ok = (x >= 1 && x <= nx - 1);
+        const int y0 = tile.y0, y1 = tile.y1;
+        const int step0 = it.step0 + ordinal * T;
+        const int k0 = ordinal * T;
+        Cell cur[T];
+        float hxp[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            cur[t] = zero_cell();
+            hxp[t] = 0.f;
+        }
+        Cell n1, n2;
+        load_cell(S, lc4, y0 - T, x, xin, wf, n1);
+        load_cell(S, lc4, y0 - T + 1, x, xin, wf, n2);
+        for (int r = y0 - T; r < y1 + T; ++r) {
+            Cell in = n1;
+            n1 = n2;
+            if (r + 2 < y1 + T) load_cell(S, lc4, r + 2, x, xin, wf, n2);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                Cell c = cur[t];
+                if (t < nst) {
+                    // wave-uniform row coefficients
+                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
+                    // H half-step with the CPML recursions
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    c.phx = bxh * c.phx + axh * dEx;
+                    c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+                    const float ty = dEx * kxh + S.dxf * c.phx;
+                    const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
+                    const bool hok = hx_ok && hy_ok;
+                    const float hy = hok ? c.hy + S.chy * ty : 0.f;
+                    const float hx = hok ? c.hx - S.chx * tx : 0.f;
+                    // E half-step with the CPML recursions and the Debye current
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    c.pex = bxe * c.pex + axe * dHy;
+                    c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+                    const float cy = dHy * kxe + S.dxf * c.pex;
+                    const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
+                    if (wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
+                    const bool eok = ex_ok && ey_ok;
+                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                    c.ez = eok ? en : 0.f;
+                    c.hx = hx;
+                    c.hy = hy;
+                    if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
+                }
+                cur[t] = in;
+                in = c;
+            }
+            const int yo = r - T;
+            if (yo >= y0 && yo < y1 && xown) store_cell(S, lc4, yo, x, wf, in);
+        }
+    }
+}
+
 #undef CP
+
+// Row-edge tiles of physical-mode simulations: the top and bottom rows
+// (y-side CPML strips, y-face NTFF rows, PEC rows, rows outside the grid) over
+// clean columns (every lane inside the grid, away from x strips and x-face
+// contour columns).  Every row-dependent choice is wave-uniform: the y-strip
+// coefficients come from the identity-padded table by a uniform index.
+struct YCell {
+    float ez, hx, hy, jp, pey, phy;
+    int mat;
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles, int ntiles,
+                                              const LaunchItem* __restrict__ items,
+                                              const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcy[6 * TOMO_MAX_NC];  // ike ikm ae be am bm with an identity entry
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    constexpr int V = kWave - 2 * T;
+    constexpr int ID = TOMO_MAX_NC - 1;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;
+        const DevSim* Sp = sims + it.sim;
+        __syncthreads();
+        {
+            float* lcf = (float*)lc4;
+            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        }
+        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+            const bool id = (i >= Sp->nc);
+            lcy[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
+            lcy[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
+            lcy[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
+            lcy[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
+            lcy[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
+            lcy[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+        }
+        __syncthreads();
+        if (wv >= tile.nband) continue;  // after the last barrier of this tile
+        const int par = it.parity0 ^ (ordinal & 1);
+        const int ps = Sp->ps, js = Sp->js, ys = Sp->ys, pitch = Sp->pitch, nx = Sp->nx, ny = Sp->ny, nc = Sp->nc;
+        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+        gfloat* jin = (gfloat*)Sp->jbase + par * js;
+        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+        gfloat* pe_in = (gfloat*)Sp->ybase + par * ys;
+        gfloat* pe_out = (gfloat*)Sp->ybase + (par ^ 1) * ys;
+        gfloat* ph_in = (gfloat*)Sp->ybase + (2 + par) * ys;
+        gfloat* ph_out = (gfloat*)Sp->ybase + (2 + (par ^ 1)) * ys;
+        gcu8* mat = (gcu8*)Sp->mat;
+        gcint* span = (gcint*)Sp->span;
+        const float chx = Sp->chx, chy = Sp->chy, dyf = Sp->dyf;
+        const int bx = tile.x0 + wv * V;
+        const int x0 = bx - T;
+        const int x = x0 + lane;
+        const bool own = lane >= T && x < min(bx + V, tile.xe);
+        const int sx = Sp->sx, sy = Sp->sy;
+        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        // y-face NTFF rows (FDTD_common.cxx:4-46 legs yn, yp)
+        const int D = Sp->D, P = Sp->P, K = Sp->chunk, L2 = Sp->L2, cx_hi = Sp->cx_hi;
+        const bool xr = (x >= D + 1 && x <= cx_hi);
+        gfloat* samp = (gfloat*)Sp->samp;
+        const int y0 = tile.y0, y1 = tile.y1;
+        const int step0 = it.step0 + ordinal * T;
+        const int k0 = ordinal * T;
+        auto load = [&](int r, YCell& c) {
+            c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
+            c.mat = 0;
+            if (r < 0 || r > ny) return;  // wave-uniform
+            const int b = r * pitch + x;
+            c.ez = fin[b];
+            c.mat = mat[b];
+            if (r < ny) {
+                c.hx = fin[2 * ps + b];
+                c.hy = fin[4 * ps + b];
+            }
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+            if (r >= 1 && r <= nc) c.pey = pe_in[(r - 1) * pitch + x];
+            else if (r >= ny - nc && r <= ny - 1) c.pey = pe_in[(nc + r - (ny - nc)) * pitch + x];
+            if (r < nc) c.phy = ph_in[r * pitch + x];
+            else if (r >= ny - nc && r < ny) c.phy = ph_in[(nc + r - (ny - nc)) * pitch + x];
+        };
+        YCell cur[T];
+        float hxp[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pey = cur[t].phy = 0.f;
+            cur[t].mat = 0;
+            hxp[t] = 0.f;
+        }
+        YCell n1, n2;
+        load(y0 - T, n1);
+        load(y0 - T + 1, n2);
+        for (int r = y0 - T; r < y1 + T; ++r) {
+            YCell in = n1;
+            n1 = n2;
+            if (r + 2 < y1 + T) load(r + 2, n2);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                YCell c = cur[t];
+                if (t < nst) {
+                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
+                    const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
+                    const float hy = hok ? c.hy + chy * dEx : 0.f;
+                    const float hx = hok ? c.hx - chx * tx : 0.f;
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
+                    const float cx = dHx * lcy[iye] + dyf * c.pey;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
+                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                    c.ez = eok ? en : 0.f;
+                    c.hx = hx;
+                    c.hy = hy;
+                    if ((y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 && y < y1 && own && xr) {
+                        const int k = k0 + t;
+                        if (y == D) {
+                            samp[k * P + x - D - 1] = c.hx;
+                            samp[(K + k) * P + x - D - 1] = c.ez;
+                        } else if (y == D + 1) {
+                            samp[(2 * K + k) * P + x - D - 1] = c.ez;
+                        } else if (y == ny - D - 1) {
+                            samp[k * P + L2 + (cx_hi - x)] = c.hx;
+                            samp[(K + k) * P + L2 + (cx_hi - x)] = c.ez;
+                        } else {
+                            samp[(2 * K + k) * P + L2 + (cx_hi - x)] = c.ez;
+                        }
+                    }
+                }
+                cur[t] = in;
+                in = c;
+            }
+            const int yo = r - T;
+            if (yo >= y0 && yo < y1 && own) {
+                const int b = yo * pitch + x;
+                fout[b] = in.ez;
+                if (yo < ny) {
+                    fout[2 * ps + b] = in.hx;
+                    fout[4 * ps + b] = in.hy;
+                }
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
+                if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = in.pey;
+                else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = in.pey;
+                if (yo < nc) ph_out[yo * pitch + x] = in.phy;
+                else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = in.phy;
+            }
+        }
+    }
+}
+
+// Column-edge tiles of physical-mode simulations: bands that touch an x-side
+// CPML strip, x-face NTFF column or x edge, over rows whose ghost zone is
+// clean (no y strip, contour row or PEC row).  The x-strip coefficients and
+// the existence masks are lane constants; every row is an x-face contour row,
+// so contour columns sample at every stage.
+struct XCell {
+    float ez, hx, hy, jp, pex, phx;
+    int mat;
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles, int ntiles,
+                                              const LaunchItem* __restrict__ items,
+                                              const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcx[6 * TOMO_MAX_NC];  // ike ikm ae be am bm with an identity entry
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    constexpr int V = kWave - 2 * T;
+    constexpr int ID = TOMO_MAX_NC - 1;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;
+        const DevSim* Sp = sims + it.sim;
+        __syncthreads();
+        {
+            float* lcf = (float*)lc4;
+            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        }
+        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+            const bool id = (i >= Sp->nc);
+            lcx[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
+            lcx[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
+            lcx[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
+            lcx[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
+            lcx[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
+            lcx[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+        }
+        __syncthreads();
+        if (wv >= tile.nband) continue;  // after the last barrier of this tile
+        const int par = it.parity0 ^ (ordinal & 1);
+        const int ps = Sp->ps, js = Sp->js, xs = Sp->xs, pitch = Sp->pitch, nx = Sp->nx, nc = Sp->nc;
+        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+        gfloat* jin = (gfloat*)Sp->jbase + par * js;
+        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+        gfloat* pe_in = (gfloat*)Sp->xbase + par * xs;
+        gfloat* pe_out = (gfloat*)Sp->xbase + (par ^ 1) * xs;
+        gfloat* ph_in = (gfloat*)Sp->xbase + (2 + par) * xs;
+        gfloat* ph_out = (gfloat*)Sp->xbase + (2 + (par ^ 1)) * xs;
+        gcu8* mat = (gcu8*)Sp->mat;
+        gcint* span = (gcint*)Sp->span;
+        const float chx = Sp->chx, chy = Sp->chy, dxf = Sp->dxf;
+        const int bx = tile.x0 + wv * V;
+        const int x0 = bx - T;
+        const int x = x0 + lane;
+        const bool xin = (x >= 0 && x <= nx);
+        const bool own = xin && lane >= T && x < min(bx + V, tile.xe);
+        const int sx = Sp->sx, sy = Sp->sy;
+        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        // x-strip membership, psi storage column, coefficients, existence masks
+        const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+        const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+        const int je = (x >= 1 && x <= nc) ? x - 1 : ((x >= nx - nc && x <= nx - 1) ? nc + x - (nx - nc) : -1);
+        const int jh = (x >= 0 && x < nc) ? x : ((x >= nx - nc && x < nx) ? nc + x - (nx - nc) : -1);
+        const float kxe = lcx[ixe], kxh = lcx[TOMO_MAX_NC + ixh];
+        const float axe = lcx[2 * TOMO_MAX_NC + ixe], bxe = lcx[3 * TOMO_MAX_NC + ixe];
+        const float axh = lcx[4 * TOMO_MAX_NC + ixh], bxh = lcx[5 * TOMO_MAX_NC + ixh];
+        const bool hok = (x >= 0 && x < nx), eok = (x >= 1 && x <= nx - 1);
+        // x-face NTFF contour columns (FDTD_common.cxx:4-46 legs xp, xn)
+        const int D = Sp->D, ny = Sp->ny, P = Sp->P, K = Sp->chunk;
+        const bool ccol = (x == nx - D - 1 || x == nx - D || x == D || x == D + 1);
+        const bool wcc = (x0 <= D + 1 && x0 + kWave - 1 >= D) || (x0 <= nx - D && x0 + kWave - 1 >= nx - D - 1);
+        gfloat* samp = (gfloat*)Sp->samp;
+        const int L1 = Sp->L1, L3 = Sp->L3, cy_lo = Sp->cy_lo;
+        const int y0 = tile.y0, y1 = tile.y1;
+        const int step0 = it.step0 + ordinal * T;
+        const int k0 = ordinal * T;
+        const int w2 = 2 * nc;
+        auto load = [&](int r, XCell& c) {
+            c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
+            c.mat = 0;
+            if (!xin) return;
+            const int b = r * pitch + x;
+            c.ez = fin[b];
+            c.mat = mat[b];
+            if (hok) {
+                c.hx = fin[2 * ps + b];
+                c.hy = fin[4 * ps + b];
+            }
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+            if (je >= 0) c.pex = pe_in[r * w2 + je];
+            if (jh >= 0) c.phx = ph_in[r * w2 + jh];
+        };
+        XCell cur[T];
+        float hxp[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pex = cur[t].phx = 0.f;
+            cur[t].mat = 0;
+            hxp[t] = 0.f;
+        }
+        XCell n1, n2;
+        load(y0 - T, n1);
+        load(y0 - T + 1, n2);
+        for (int r = y0 - T; r < y1 + T; ++r) {
+            XCell in = n1;
+            n1 = n2;
+            if (r + 2 < y1 + T) load(r + 2, n2);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                XCell c = cur[t];
+                if (t < nst) {
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    c.phx = bxh * c.phx + axh * dEx;
+                    const float ty = dEx * kxh + dxf * c.phx;
+                    const float hy = hok ? c.hy + chy * ty : 0.f;
+                    const float hx = hok ? c.hx - chx * dEy : 0.f;
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    c.pex = bxe * c.pex + axe * dHy;
+                    const float cy = dHy * kxe + dxf * c.pex;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * cy - qa.y * dHx - qb.x * c.jp;
+                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                    c.ez = eok ? en : 0.f;
+                    c.hx = hx;
+                    c.hy = hy;
+                    if (wcc && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
+                        const int k = k0 + t;
+                        if (x == nx - D - 1) {
+                            samp[k * P + L1 + y - cy_lo] = c.hy;
+                            samp[(K + k) * P + L1 + y - cy_lo] = c.ez;
+                        } else if (x == nx - D) {
+                            samp[(2 * K + k) * P + L1 + y - cy_lo] = c.ez;
+                        } else if (x == D) {
+                            samp[k * P + L3 + (ny - D - 1 - y)] = c.hy;
+                            samp[(K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
+                        } else {
+                            samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
+                        }
+                    }
+                }
+                cur[t] = in;
+                in = c;
+            }
+            const int yo = r - T;
+            if (yo >= y0 && yo < y1 && own) {
+                const int b = yo * pitch + x;
+                fout[b] = in.ez;
+                if (hok) {
+                    fout[2 * ps + b] = in.hx;
+                    fout[4 * ps + b] = in.hy;
+                }
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
+                if (je >= 0) pe_out[yo * w2 + je] = in.pex;
+                if (jh >= 0) ph_out[yo * w2 + jh] = in.phx;
+            }
+        }
+    }
+}
+
+// Interior tiles of physical-mode simulations: every lane is inside the
+// grid, no CPML strip, NTFF contour row/column or PEC edge lies within the
+// tile's ghost zone, so each stage is the bare Yee/Debye update (plus the
+// point source when the source cell is in the tile).  Two rows are kept in
+// flight ahead of the sweep to cover HBM latency.
+struct ICell {
+    float ez, hx, hy, jp;
+    int mat;
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ tiles, int ntiles,
+                                                const LaunchItem* __restrict__ items,
+                                                const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    constexpr int V = kWave - 2 * T;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;
+        const DevSim* Sp = sims + it.sim;
+        __syncthreads();
+        {
+            float* lcf = (float*)lc4;
+            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        }
+        __syncthreads();
+        if (wv >= tile.nband) continue;  // after the last barrier of this tile
+        const int par = it.parity0 ^ (ordinal & 1);
+        const int ps = Sp->ps, js = Sp->js, pitch = Sp->pitch;
+        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+        gfloat* jin = (gfloat*)Sp->jbase + par * js;
+        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+        gcu8* mat = (gcu8*)Sp->mat;
+        gcint* span = (gcint*)Sp->span;
+        const float chx = Sp->chx, chy = Sp->chy;
+        const int x0 = tile.x0 + wv * V - T;
+        const int x = x0 + lane;
+        const bool own = lane >= T && lane < kWave - T;
+        const int sx = Sp->sx, sy = Sp->sy;
+        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        const int y0 = tile.y0, y1 = tile.y1;
+        const int step0 = it.step0 + ordinal * T;
+        auto load = [&](int r, ICell& c) {
+            const int b = r * pitch + x;
+            c.ez = fin[b];
+            c.hx = fin[2 * ps + b];
+            c.hy = fin[4 * ps + b];
+            c.mat = mat[b];
+            c.jp = 0.f;
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+        };
+        ICell cur[T];
+        float hxp[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = 0.f;
+            cur[t].mat = 0;
+            hxp[t] = 0.f;
+        }
+        ICell n1, n2;
+        load(y0 - T, n1);
+        load(y0 - T + 1, n2);
+        for (int r = y0 - T; r < y1 + T; ++r) {
+            ICell in = n1;
+            n1 = n2;
+            if (r + 2 < y1 + T) load(r + 2, n2);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                ICell c = cur[t];
+                if (t < nst) {
+                    // H half-step
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    const float hy = c.hy + chy * dEx;
+                    const float hx = c.hx - chx * dEy;
+                    // E half-step with the Debye polarisation current
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
+                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                    c.jp = qb.y * c.jp + qb.z * (en - c.ez);
+                    c.ez = en;
+                    c.hx = hx;
+                    c.hy = hy;
+                }
+                cur[t] = in;
+                in = c;
+            }
+            const int yo = r - T;
+            if (yo >= y0 && yo < y1 && own) {
+                const int b = yo * pitch + x;
+                fout[b] = in.ez;
+                fout[2 * ps + b] = in.hx;
+                fout[4 * ps + b] = in.hy;
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
+            }
+        }
+    }
+}
 
 // ---------------------------------------------------------------- running DFT fold
 // Phase e^{-j w (step + t0) dt}: reseeded in FP64 at the chunk start, advanced
@@ -610,20 +1263,72 @@ __global__ void k_zero(uint4* __restrict__ p, size_t n16) {
 bool tblock_supported(int T) { return T == 1 || T == 2 || T == 4 || T == 6 || T == 8; }
 
 hipError_t launch_tblock(int T, int mode, const TileItem* tiles, int ntiles, const LaunchItem* items,
-                         const DevSim* sims, int ordinal, int seg_rows, int waves, int grid, hipStream_t st) {
+                         const DevSim* sims, int ordinal, int waves, int grid, hipStream_t st) {
     dim3 g(grid), b(waves * kWave);
 #define TB(TT)                                                                                                 \
     if (T == TT) {                                                                                             \
         if (mode == TOMO_MODE_REF_COMPAT)                                                                      \
             hipLaunchKernelGGL((k_tblock<TT, TOMO_MODE_REF_COMPAT>), g, b, 0, st, tiles, ntiles, items, sims,   \
-                               ordinal, seg_rows);                                                             \
+                               ordinal);                                                                       \
         else                                                                                                   \
             hipLaunchKernelGGL((k_tblock<TT, TOMO_MODE_PHYSICAL>), g, b, 0, st, tiles, ntiles, items, sims,     \
-                               ordinal, seg_rows);                                                             \
+                               ordinal);                                                                       \
         return hipGetLastError();                                                                              \
     }
     TB(1) TB(2) TB(4) TB(6) TB(8)
 #undef TB
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tedge(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                        int ordinal, int waves, int grid, hipStream_t st) {
+    dim3 g(grid), b(waves * kWave);
+#define TE(TT)                                                                                       \
+    if (T == TT) {                                                                                   \
+        hipLaunchKernelGGL((k_tedge<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);         \
+        return hipGetLastError();                                                                    \
+    }
+    TE(1) TE(2) TE(4) TE(6) TE(8)
+#undef TE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_trow(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                       int ordinal, int waves, int grid, hipStream_t st) {
+    dim3 g(grid), b(waves * kWave);
+#define TR(TT)                                                                                       \
+    if (T == TT) {                                                                                   \
+        hipLaunchKernelGGL((k_trow<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
+        return hipGetLastError();                                                                    \
+    }
+    TR(1) TR(2) TR(4) TR(6) TR(8)
+#undef TR
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tcol(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                       int ordinal, int waves, int grid, hipStream_t st) {
+    dim3 g(grid), b(waves * kWave);
+#define TC(TT)                                                                                       \
+    if (T == TT) {                                                                                   \
+        hipLaunchKernelGGL((k_tcol<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
+        return hipGetLastError();                                                                    \
+    }
+    TC(1) TC(2) TC(4) TC(6) TC(8)
+#undef TC
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                         int ordinal, int waves, int grid, hipStream_t st) {
+    dim3 g(grid), b(waves * kWave);
+#define TI(TT)                                                                                       \
+    if (T == TT) {                                                                                   \
+        hipLaunchKernelGGL((k_tinner<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
+        return hipGetLastError();                                                                    \
+    }
+    TI(1) TI(2) TI(4) TI(6) TI(8)
+#undef TI
     return hipErrorInvalidValue;
 }
 

@@ -53,6 +53,10 @@ struct Slot {
     int target = 0;     // steps to reach in the current run
     long req = -1;      // owning request (serving)
     int exc = 0;
+    uint64_t serial = 0; // unique per simulation (table indices are reused)
+    // cell accounting of the current tile split (for kernel timing)
+    double inner_nodes = 0, inner_jp = 0;
+    std::vector<float> hpos;  // host copy of the contour positions (source of an async upload)
 };
 
 // Bytes one simulation slot occupies on the device.
@@ -92,7 +96,7 @@ struct tomo_ctx {
     hipStream_t stream = nullptr;
     int cus = 256;
     uint64_t budget = 0, used = 0;
-    int T = 4, S = 32, W = 8, K = 32;  // schedule
+    int T = 4, S = 32, W = 4, K = 32;  // schedule (W: waves per edge workgroup)
     int grid_mult = 4;                 // persistent workgroups per CU
     bool use_graph = true;
     // DevSim table
@@ -107,9 +111,13 @@ struct tomo_ctx {
     TileItem* d_tiles = nullptr;
     int tiles_cap = 0;
     hipEvent_t staged = nullptr;
-    std::vector<int> tile_key;  // slot table indices the uploaded tile list was built for
-    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0;
-    int ntiles = 0;
+    std::vector<uint64_t> tile_key;  // serials of the simulations the uploaded tile list was built for
+    uint64_t next_serial = 1;
+    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0;
+    int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
+    int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
+    int n_corner = 0;
+    int Wi = 4;                  // waves per workgroup of the interior kernel
     // serving
     std::mutex mu;
     uint64_t next_ticket = 1;
@@ -118,6 +126,20 @@ struct tomo_ctx {
     std::vector<Slot*> active;
     // physical-mode targets, keyed by configuration bytes
     std::map<std::string, std::vector<float>> targets;
+    // device memory of destroyed slots, kept for reuse by size
+    std::map<size_t, std::vector<char*>> cache;
+    size_t cached_bytes = 0;
+    // side streams: the three edge-tile kernels run beside the interior one;
+    // ev_side[k] marks side stream k's block done, ev_bar all of a block done
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_bar = nullptr;
+    // kernel timing
+    bool prof = false;
+    tomo_stats stats{};
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { hipEvent_t a, b; int kind; double cells, bytes; };
+    std::vector<Pending> pending;
 };
 
 namespace {
@@ -167,6 +189,14 @@ int ensure_staging(tomo_ctx* c, int nitems, int ntiles) {
     return TOMO_OK;
 }
 
+void release_cache(tomo_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->cache)
+        for (char* p : kv.second) (void)hipFree(p);
+    c->cache.clear();
+    c->cached_bytes = 0;
+}
+
 // Build the device-resident simulation for one excitation of one phantom.
 int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t np, int exc, Slot** out) {
     std::unique_ptr<Slot> s(new Slot());
@@ -176,7 +206,18 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     if (c->K % c->T) return fail(TOMO_E_INVALID, "chunk must be a multiple of the fused step count");
     const size_t bytes = slot_bytes(su, c->K);
     if (c->used + bytes > c->budget) return fail(TOMO_E_NOMEM, "HBM budget exceeded");
-    HIPCHK(hipMalloc(&s->mem, bytes));
+    auto hit = c->cache.find(bytes);
+    if (hit != c->cache.end() && !hit->second.empty()) {
+        s->mem = hit->second.back();
+        hit->second.pop_back();
+        c->cached_bytes -= bytes;
+    } else {
+        if (hipMalloc(&s->mem, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            release_cache(c);
+            HIPCHK(hipMalloc(&s->mem, bytes));
+        }
+    }
     s->bytes = bytes;
     c->used += bytes;
     rc = ensure_table(c, 1);
@@ -187,6 +228,7 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     }
     s->idx = c->free_idx.back();
     c->free_idx.pop_back();
+    s->serial = c->next_serial++;
 
     const tomo_sim_config& cf = su.cfg;
     const size_t rows = (size_t)cf.ny + 1, pitch = su.pitch, nc = cf.ncpml, P = su.P;
@@ -202,6 +244,17 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     for (int k = 0; k < 2; ++k) d.pey[k] = (float*)take(2 * nc * pitch * 4);
     for (int k = 0; k < 2; ++k) d.phy[k] = (float*)take(2 * nc * pitch * 4);
     char* state_end = m;
+    d.fbase = d.ez[0];
+    d.ps = (int)(d.ez[1] - d.ez[0]);
+    d.jbase = d.jp[0];
+    d.js = (int)(d.jp[1] - d.jp[0]);
+    d.xbase = d.pex[0];
+    d.xs = (int)(d.pex[1] - d.pex[0]);
+    d.ybase = d.pey[0];
+    d.ys = (int)(d.pey[1] - d.pey[0]);
+    if (d.hx[0] != d.fbase + 2 * (size_t)d.ps || d.hy[1] != d.fbase + 5 * (size_t)d.ps ||
+        d.phx[1] != d.xbase + 3 * (size_t)d.xs || d.phy[1] != d.ybase + 3 * (size_t)d.ys)
+        return fail(TOMO_E_INVALID, "internal: slot planes are not equally strided");
     uint8_t* mat = (uint8_t*)take(rows * pitch);
     int* span = (int*)take(3 * rows * 4);
     d.samp = (float*)take(3 * (size_t)c->K * P * 4);
@@ -239,7 +292,8 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     d.sgn_dt[0] = dtf; d.sgn_dt[1] = dtf; d.sgn_dt[2] = -dtf; d.sgn_dt[3] = -dtf;
 
     // contour positions (N2FPostProcess geometry, FDTD_GPU.cu:584-621)
-    std::vector<float> hpos(2 * P);
+    std::vector<float>& hpos = s->hpos;
+    hpos.resize(2 * P);
     {
         const int D = cf.ntff_offset, nx = cf.nx, ny = cf.ny;
         for (int p = 0; p < (int)P; ++p) {
@@ -282,32 +336,178 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
         HIPCHK(hipMemcpyAsync(yc, su.ref.yc.data(), rows * 4, hipMemcpyHostToDevice, st));
         HIPCHK(launch_raster_ref(mat, su.pitch, cf.nx, cf.ny, lo, hi, xc, yc, cf.numpatches, st));
     }
-    HIPCHK(hipMemcpyAsync(c->d_sims + s->idx, &d, sizeof(DevSim), hipMemcpyHostToDevice, st));
-    // the host copies above read pageable memory owned by `s`/`hpos`: wait before they go away
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpyAsync(c->d_sims + s->idx, &s->d, sizeof(DevSim), hipMemcpyHostToDevice, st));
+    // the uploads above read host memory owned by the slot, which outlives them
+    // (slot_destroy waits for the stream before releasing it)
     s->exc = exc;
     s->target = cf.steps;
     *out = s.release();
     return TOMO_OK;
 }
 
+// Return a slot's device memory to the cache (work still queued on the
+// stream may use it: it is handed out again only to later stream work).
 void slot_destroy(tomo_ctx* c, Slot* s) {
     if (!s) return;
     if (s->mem) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipFree(s->mem);
+        c->cache[s->bytes].push_back(s->mem);
+        c->cached_bytes += s->bytes;
         c->used -= s->bytes;
+        if (c->used + c->cached_bytes > c->budget) release_cache(c);
     }
     if (s->idx >= 0) c->free_idx.push_back(s->idx);
+    // the host staging in s (span, waveform, positions) may still be read by
+    // queued uploads: wait before releasing it
+    (void)hipStreamSynchronize(c->stream);
     delete s;
 }
 
 int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); }
 
+// Tiles of one simulation.  Columns and rows whose ghost zone touches a PEC
+// edge, a CPML strip or an NTFF contour row/column go to the edge kernel;
+// full-width bands of the clean interior go to the interior kernel (physical
+// mode only).  Interior bands start exactly where the clean zone starts, so
+// the edge frame is one narrow band wide on each side.
+void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<TileItem>& edge,
+                 std::vector<TileItem>& rowedge, std::vector<TileItem>& coledge, std::vector<TileItem>& inner) {
+    const DevSim& d = sl->d;
+    sl->inner_nodes = 0;
+    sl->inner_jp = 0;
+    const int nx = d.nx, ny = d.ny, nc = d.nc, D = d.D, V = 64 - 2 * T;
+    // column ranges [a, b) split into bands of V and grouped w bands per tile
+    auto add = [&](std::vector<TileItem>& v, int a, int b, int y0, int y1, int w) {
+        for (int x = a; x < b; x += w * V) {
+            TileItem t{};
+            t.item = item;
+            t.x0 = x;
+            t.xe = std::min(x + w * V, b);
+            t.nband = (t.xe - x + V - 1) / V;
+            t.y0 = y0;
+            t.y1 = y1;
+            v.push_back(t);
+        }
+    };
+    // clean rows / columns lie strictly between the special ones
+    const int ybot = std::max(nc, D + 1) + 1, ytop = std::min(ny - nc, ny - D - 1) - 1;
+    const int xbot = std::max(nc, D + 1) + 1, xtop = std::min(nx - nc, nx - D - 1) - 1;
+    const int ixl = xbot + T;                         // first interior owned column
+    const int nbi = std::max(0, (xtop - T + 1 - ixl) / V);  // full interior bands
+    const int ixr = ixl + nbi * V;                    // end of interior owned columns
+    const int ia = ybot + T, ib = ytop - T + 1;       // interior output rows [ia, ib)
+    const bool split = d.mode == TOMO_MODE_PHYSICAL && nbi > 0 && ib > ia;
+    auto edge_rows = [&](int y0, int y1) {
+        for (int a = y0; a < y1; a += S) add(edge, 0, nx + 1, a, std::min(a + S, y1), W);
+    };
+    if (!split) {
+        edge_rows(0, ny + 1);
+        return;
+    }
+    // top and bottom rows: corner bands to the general kernel, clean-column
+    // bands (full width, lanes inside [xbot, xtop]) to the row-edge kernel
+    auto edge_rows_split = [&](int y0, int y1) {
+        const int sc = std::min(S, 8);  // corner tiles: short row chunks (their code path is the slowest)
+        for (int a = y0; a < y1; a += sc) {
+            const int a1 = std::min(a + sc, y1);
+            add(edge, 0, ixl, a, a1, W);
+            add(edge, ixr, nx + 1, a, a1, W);
+        }
+        for (int a = y0; a < y1; a += S) add(rowedge, ixl, ixr, a, std::min(a + S, y1), Wi);
+    };
+    edge_rows_split(0, ia);
+    for (int a = ia; a < ib; a += S) {
+        const int a1 = std::min(a + S, ib);
+        add(coledge, 0, ixl, a, a1, W);
+        add(inner, ixl, ixr, a, a1, Wi);
+        add(coledge, ixr, nx + 1, a, a1, W);
+        for (int y = a; y < a1; ++y) {
+            const int lo = std::max(ixl, sl->su.span[3 * y]), hi = std::min(ixr, sl->su.span[3 * y + 1]);
+            sl->inner_jp += std::max(0, hi - lo);
+        }
+        sl->inner_nodes += (double)(ixr - ixl) * (a1 - a);
+    }
+    edge_rows_split(ib, ny + 1);
+}
+
+// Algorithmic HBM bytes per cell-step of one simulation: Ez, Hx, Hy read and
+// written once (24), the material id (1), Jp on Debye cells (8 f_disp) and the
+// psi pair on CPML-strip cells (16 f_strip).
+double alg_bytes_per_cell_step(const Slot* s) {
+    const double nx = s->d.nx, ny = s->d.ny, nc = s->d.nc;
+    const double cells = nx * ny;
+    const double f_disp = (double)s->su.jp_len / cells;
+    const double f_strip = (2 * nc * ny + 2 * nc * nx - 4 * nc * nc) / cells;
+    return 24.0 + 1.0 + 8.0 * f_disp + 16.0 * f_strip;
+}
+
+hipEvent_t pool_event(tomo_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Resolve recorded launch timings (synchronises the stream).
+int drain_timings(tomo_ctx* c) {
+    if (c->pending.empty()) return TOMO_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) HIPCHK(hipStreamSynchronize(c->side[k]));
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+        if (p.kind == 0) {
+            c->stats.tblock_launches++;
+            c->stats.tblock_ms += ms;
+            c->stats.cell_steps += p.cells;
+            c->stats.alg_bytes += p.bytes;
+        } else if (p.kind == 4) {
+            c->stats.tblock_ms += ms;
+        } else if (p.kind == 3) {
+            c->stats.inner_launches++;
+            c->stats.inner_ms += ms;
+            c->stats.inner_cell_steps += p.cells;
+            c->stats.inner_alg_bytes += p.bytes;
+        } else if (p.kind == 1) {
+            c->stats.fold_launches++;
+            c->stats.fold_ms += ms;
+        } else {
+            c->stats.project_launches++;
+            c->stats.project_ms += ms;
+        }
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->pending.clear();
+    return TOMO_OK;
+}
+
+struct LaunchTimer {
+    tomo_ctx* c;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(tomo_ctx* c_, hipStream_t s_ = nullptr) : c(c_), st(s_ ? s_ : c_->stream) {
+        if (c->prof) {
+            a = pool_event(c);
+            b = pool_event(c);
+            if (a) (void)hipEventRecord(a, st);
+        }
+    }
+    void done(int kind, double cells = 0, double bytes = 0) {
+        if (c->prof && a && b) {
+            (void)hipEventRecord(b, st);
+            c->pending.push_back({a, b, kind, cells, bytes});
+        }
+    }
+};
+
 // Run epochs until every slot in `sl` reaches its target step.
 int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
     const int T = c->T, S = c->S, W = c->W, K = c->K;
-    if (!tblock_supported(T) || K % T || K > 64 || W < 1 || W > 8 || S < 1)
+    if (!tblock_supported(T) || K % T || K > 64 || W < 1 || W > 4 || S < 1)
         return fail(TOMO_E_INVALID, "unsupported schedule");
     for (Slot* s : sl)
         if (s->d.chunk != K) return fail(TOMO_E_STATE, "simulation was created under a different DFT chunk");
@@ -319,13 +519,15 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
         for (Slot* s : ms) (s->d.flags & TOMO_FLAG_DFT_FP64) ? fp64_any = true : fp32_any = true;
         while (!ms.empty()) {
             // tile list (rebuilt only when the set of simulations changes)
-            std::vector<int> key;
-            for (Slot* s : ms) key.push_back(s->idx);
-            int ntiles = 0;
-            for (Slot* s : ms) {
-                const int nb = n_bands(s->d.nx, T), ng = (nb + W - 1) / W, ns = (s->d.ny + 1 + S - 1) / S;
-                ntiles += ng * ns;
+            std::vector<uint64_t> key;
+            for (Slot* s : ms) key.push_back(s->serial);
+            std::vector<TileItem> te, tr, tc, tn;
+            const bool rebuild = key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S ||
+                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi;
+            if (rebuild) {
+                for (size_t i = 0; i < ms.size(); ++i) build_tiles(ms[i], (int)i, T, S, W, c->Wi, te, tr, tc, tn);
             }
+            const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : c->ntiles;
             int rc = ensure_staging(c, (int)ms.size(), ntiles);
             if (rc) return rc;
             HIPCHK(hipEventSynchronize(c->staged));
@@ -342,27 +544,94 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             }
             HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * ms.size(), hipMemcpyHostToDevice,
                                   c->stream));
-            if (key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S || c->tile_key_W != W) {
-                int t = 0;
-                for (size_t i = 0; i < ms.size(); ++i) {
-                    Slot* s = ms[i];
-                    const int nb = n_bands(s->d.nx, T), ng = (nb + W - 1) / W, ns = (s->d.ny + 1 + S - 1) / S;
-                    for (int g = 0; g < ns; ++g)
-                        for (int b = 0; b < ng; ++b) c->h_tiles[t++] = TileItem{(int)i, b * W, g, 0};
-                }
+            if (rebuild) {
+                TileItem* h = c->h_tiles;
+                h = std::copy(te.begin(), te.end(), h);
+                h = std::copy(tr.begin(), tr.end(), h);
+                h = std::copy(tc.begin(), tc.end(), h);
+                std::copy(tn.begin(), tn.end(), h);
                 HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, sizeof(TileItem) * ntiles, hipMemcpyHostToDevice,
                                       c->stream));
                 c->tile_key = key;
-                c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W;
+                c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W; c->tile_key_Wi = c->Wi;
                 c->ntiles = ntiles;
+                c->n_edge = (int)(te.size() + tr.size() + tc.size());
+                c->n_corner = (int)te.size();
+                c->n_rowedge = (int)tr.size();
             }
             HIPCHK(hipEventRecord(c->staged, c->stream));
-            const int grid = std::max(1, std::min(ntiles, c->cus * c->grid_mult));
+            const int n_edge = c->n_edge, n_inner = c->ntiles - c->n_edge;
+            const int cap = c->cus * c->grid_mult;
             const int nl = (maxn + T - 1) / T;
-            for (int o = 0; o < nl; ++o)
-                HIPCHK(launch_tblock(T, mode, c->d_tiles, ntiles, c->d_items, c->d_sims, o, S, W, grid, c->stream));
-            if (fp32_any) HIPCHK(launch_fold(false, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
-            if (fp64_any) HIPCHK(launch_fold(true, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+            // tile classes: corners (general kernel), row-edge, column-edge, interior
+            const int n0 = c->n_corner, nr = c->n_rowedge, ncl = n_edge - n0 - nr;
+            const int cnt[3] = {n0, nr, ncl};
+            int nside = 0;
+            for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
+            const bool multi = mode == TOMO_MODE_PHYSICAL && n_inner > 0 && nside > 0;
+            if (multi) HIPCHK(hipEventRecord(c->ev_bar, c->stream));
+            for (int o = 0; o < nl; ++o) {
+                double ecells = 0, ebytes = 0, icells = 0, ibytes = 0;
+                if (c->prof) {
+                    for (size_t i = 0; i < ms.size(); ++i) {
+                        const int nst = std::min(T, c->h_items[i].nsteps - o * T);
+                        if (nst <= 0) continue;
+                        const Slot* q = ms[i];
+                        const double all_c = (double)q->d.nx * q->d.ny;
+                        const double all_b = all_c * alg_bytes_per_cell_step(q);
+                        const double in_b = q->inner_nodes * 25.0 + q->inner_jp * 8.0;
+                        icells += nst * q->inner_nodes;
+                        ibytes += nst * in_b;
+                        ecells += nst * (all_c - q->inner_nodes);
+                        ebytes += nst * (all_b - in_b);
+                    }
+                }
+                if (mode != TOMO_MODE_PHYSICAL) {
+                    LaunchTimer lt(c, c->stream);
+                    HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
+                                         std::min(n_edge, cap), c->stream));
+                    lt.done(0, ecells, ebytes);
+                    continue;
+                }
+                // every class of block o starts after all of block o-1 (ev_bar)
+                int off = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const int n = cnt[k];
+                    const TileItem* tl = c->d_tiles + off;
+                    off += n;
+                    if (n == 0) continue;
+                    hipStream_t es = multi ? c->side[k] : c->stream;
+                    if (multi) HIPCHK(hipStreamWaitEvent(es, c->ev_bar, 0));
+                    LaunchTimer lt(c, es);
+                    if (k == 0) HIPCHK(launch_tedge(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                    if (k == 1) HIPCHK(launch_trow(T, tl, n, c->d_items, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
+                    if (k == 2) HIPCHK(launch_tcol(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                    // the three edge launches of a block share one timing record (kind 0)
+                    lt.done(k == 0 ? 0 : 4, k == 0 ? ecells : 0, k == 0 ? ebytes : 0);
+                    if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
+                }
+                if (n_inner > 0) {
+                    LaunchTimer lt(c, c->stream);
+                    HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, c->d_items, c->d_sims, o, c->Wi,
+                                         std::min(n_inner, 2 * cap), c->stream));
+                    lt.done(3, icells, ibytes);
+                }
+                if (multi) {
+                    for (int k = 0; k < 3; ++k)
+                        if (cnt[k] > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side[k], 0));
+                    HIPCHK(hipEventRecord(c->ev_bar, c->stream));
+                }
+            }
+            {
+                LaunchTimer lt(c);
+                if (fp32_any) HIPCHK(launch_fold(false, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+                if (fp64_any) HIPCHK(launch_fold(true, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+                lt.done(1);
+            }
+            if (c->pending.size() > 4096) {
+                int rc2 = drain_timings(c);
+                if (rc2) return rc2;
+            }
             std::vector<Slot*> next;
             for (size_t i = 0; i < ms.size(); ++i) {
                 Slot* s = ms[i];
@@ -416,7 +685,9 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
     ProjItem* d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(ProjItem) * items.size()));
     HIPCHK(hipMemcpyAsync(d, items.data(), sizeof(ProjItem) * items.size(), hipMemcpyHostToDevice, c->stream));
+    LaunchTimer lt(c);
     hipError_t e = launch_project(d, (int)items.size(), c->d_sims, nullptr, A, (float)eta0(), dD, dP, c->stream);
+    lt.done(2);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipFree(d));
     HIPCHK(e);
@@ -708,6 +979,11 @@ int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx** out) {
     std::unique_ptr<tomo_ctx> c(new tomo_ctx());
     c->device = device;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 3; ++k) {
+        HIPCHK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_side[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&c->ev_bar, hipEventDisableTiming));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -737,31 +1013,139 @@ int tomo_ctx_destroy(tomo_ctx* c) {
     if (c->h_tiles) (void)hipHostFree(c->h_tiles);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     if (c->staged) (void)hipEventDestroy(c->staged);
+    (void)drain_timings(c);
+    release_cache(c);
+    for (int k = 0; k < 3; ++k) {
+        if (c->ev_side[k]) (void)hipEventDestroy(c->ev_side[k]);
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    }
+    if (c->ev_bar) (void)hipEventDestroy(c->ev_bar);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TOMO_OK;
 }
 
-int tomo_set_schedule(tomo_ctx* c, int tsteps, int seg_rows, int waves, int chunk) {
-    if (!c) return fail(TOMO_E_INVALID, "null context");
-    std::lock_guard<std::mutex> g(c->mu);
-    if (!c->active.empty()) return fail(TOMO_E_STATE, "schedule cannot change while requests are resident");
-    int T = tsteps ? tsteps : c->T, S = seg_rows ? seg_rows : c->S, W = waves ? waves : c->W,
-        K = chunk ? chunk : c->K;
+static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
+    const int T = s.tsteps ? s.tsteps : c->T, S = s.seg_rows ? s.seg_rows : c->S;
+    const int W = s.edge_waves ? s.edge_waves : c->W, Wi = s.inner_waves ? s.inner_waves : c->Wi;
+    const int K = s.chunk ? s.chunk : c->K, G = s.grid_mult ? s.grid_mult : c->grid_mult;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
-    if (W < 1 || W > 8 || S < 4 || S > 4096 || K < T || K > 64 || K % T)
+    if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64)
         return fail(TOMO_E_INVALID, "bad schedule");
-    c->T = T; c->S = S; c->W = W; c->K = K;
+    c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G;
     return TOMO_OK;
 }
 
-int tomo_get_schedule(tomo_ctx* c, int* t, int* s, int* w, int* k) {
-    if (!c) return fail(TOMO_E_INVALID, "null context");
-    if (t) *t = c->T;
-    if (s) *s = c->S;
-    if (w) *w = c->W;
-    if (k) *k = c->K;
+int tomo_set_schedule(tomo_ctx* c, const tomo_schedule* s) {
+    if (!c || !s) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->active.empty()) return fail(TOMO_E_STATE, "schedule cannot change while requests are resident");
+    if (s->chunk && s->chunk != c->K && c->cap - (int)c->free_idx.size() > 0)
+        return fail(TOMO_E_STATE, "the DFT chunk cannot change while simulations are resident");
+    return set_schedule_locked(c, *s);
+}
+
+int tomo_get_schedule(tomo_ctx* c, tomo_schedule* s) {
+    if (!c || !s) return fail(TOMO_E_INVALID, "null argument");
+    s->tsteps = c->T; s->seg_rows = c->S; s->edge_waves = c->W; s->inner_waves = c->Wi;
+    s->chunk = c->K; s->grid_mult = c->grid_mult;
     return TOMO_OK;
+}
+
+// Seconds for `epochs` epochs of the resident slots under the current schedule.
+static int time_epochs(tomo_ctx* c, std::vector<Slot*>& sl, int epochs, double* secs) {
+    hipEvent_t a = pool_event(c), b = pool_event(c);
+    if (!a || !b) return fail(TOMO_E_HIP, "event");
+    for (Slot* s : sl) s->target = std::min(s->su.cfg.steps, s->done + c->K);
+    int rc = run_epochs(c, sl);  // untimed: rebuilds and uploads the tile list
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(a, c->stream));
+    for (int e = 0; e < epochs; ++e) {
+        for (Slot* s : sl) s->target = std::min(s->su.cfg.steps, s->done + c->K);
+        rc = run_epochs(c, sl);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    c->ev_pool.push_back(a);
+    c->ev_pool.push_back(b);
+    *secs = ms / 1e3;
+    return TOMO_OK;
+}
+
+int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_schedule* best, double* rate) {
+    if (!c || !cfg || n_sims < 1) return fail(TOMO_E_INVALID, "bad argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->active.empty()) return fail(TOMO_E_STATE, "autotune needs an idle context");
+    tomo_sim_config tc = *cfg;
+    tc.steps = 1 << 17;  // never runs out during the trials
+    std::vector<double> p;
+    if (tc.mode == TOMO_MODE_PHYSICAL) {
+        p = target_vector(tc);
+    } else {
+        const int n2 = tc.numpatches * tc.numpatches;
+        p.assign((size_t)3 * n2, 0.0);
+        for (int i = 0; i < n2; ++i) { p[i] = 3.0; p[n2 + i] = 10.0; }
+    }
+    std::vector<Slot*> sl;
+    int rc = TOMO_OK;
+    for (int i = 0; i < n_sims && !rc; ++i) {
+        Slot* s = nullptr;
+        rc = slot_create(c, tc, p.data(), p.size(), i % tc.n_exc, &s);
+        if (!rc) sl.push_back(s);
+    }
+    const bool prof = c->prof;
+    c->prof = false;
+    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult};
+    double cells = 0;
+    for (Slot* s : sl) cells += (double)s->d.nx * s->d.ny * c->K;
+    double best_t = 1e30;
+    auto trial = [&](const tomo_schedule& cand) -> int {
+        int r = set_schedule_locked(c, cand);
+        if (r) return r;
+        double t = 0;
+        r = time_epochs(c, sl, 2, &t);
+        if (r) return r;
+        if (t < best_t) { best_t = t; cur = cand; }
+        return TOMO_OK;
+    };
+    if (!rc) rc = trial(cur);
+    const int Ts[] = {1, 2, 4, 8}, Ss[] = {8, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+        for (int v : Ts) { tomo_schedule x = cur; x.tsteps = v; if (!rc && c->K % v == 0) rc = trial(x); }
+        for (int v : Ss) { tomo_schedule x = cur; x.seg_rows = v; if (!rc) rc = trial(x); }
+        for (int v : Ws) { tomo_schedule x = cur; x.edge_waves = v; if (!rc) rc = trial(x); }
+        for (int v : Ws) { tomo_schedule x = cur; x.inner_waves = v; if (!rc) rc = trial(x); }
+        for (int v : Gs) { tomo_schedule x = cur; x.grid_mult = v; if (!rc) rc = trial(x); }
+    }
+    if (!rc) rc = set_schedule_locked(c, cur);
+    c->prof = prof;
+    for (Slot* s : sl) slot_destroy(c, s);
+    c->tile_key.clear();
+    if (rc) return rc;
+    if (best) *best = cur;
+    if (rate) *rate = cells * 2 / best_t;
+    return TOMO_OK;
+}
+
+int tomo_profile(tomo_ctx* c, int enable) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = drain_timings(c);
+    c->prof = enable != 0;
+    c->stats = tomo_stats{};
+    return rc;
+}
+
+int tomo_get_stats(tomo_ctx* c, tomo_stats* out) {
+    if (!c || !out) return fail(TOMO_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    int rc = drain_timings(c);
+    *out = c->stats;
+    return rc;
 }
 
 int tomo_eval(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n_params, const float* target,

@@ -15,6 +15,9 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 
 namespace tomo {
 
@@ -132,7 +135,7 @@ int preset(int id, int grid, tomo_sim_config* out) {
 int validate(const tomo_sim_config& c) {
     if (c.mode != TOMO_MODE_REF_COMPAT && c.mode != TOMO_MODE_PHYSICAL) return fail(TOMO_E_INVALID, "bad mode");
     if (c.nx < 16 || c.ny < 16 || c.nx > 32768 || c.ny > 32768) return fail(TOMO_E_INVALID, "grid size out of range");
-    if (c.ncpml < 1 || c.ncpml > TOMO_MAX_NC || 2 * c.ncpml + 4 > c.nx || 2 * c.ncpml + 4 > c.ny)
+    if (c.ncpml < 1 || c.ncpml > TOMO_MAX_NC - 1 || 2 * c.ncpml + 4 > c.nx || 2 * c.ncpml + 4 > c.ny)
         return fail(TOMO_E_INVALID, "bad CPML thickness");
     if (c.n_freq < 1 || c.n_freq > TOMO_MAX_FREQ) return fail(TOMO_E_INVALID, "n_freq out of range");
     if (c.n_angles < 1 || c.n_angles > 4096) return fail(TOMO_E_INVALID, "n_angles out of range");
@@ -411,6 +414,35 @@ static void phys_cpml(const tomo_sim_config& c, SimSetup& s) {
     }
 }
 
+// Products of the configuration alone (shared by every simulation of it):
+// the physical source waveform and its spectrum normaliser, the reference
+// disc extents and patch coordinates.
+struct ConfigProducts {
+    std::vector<float> srcw;
+    double inv_norm[TOMO_MAX_FREQ] = {};
+    RefRaster ref;
+    std::vector<int> span;
+    int jp_len = 0;
+    bool has_ref = false, has_phys = false;
+};
+
+static std::mutex g_cp_mu;
+static std::map<std::string, std::shared_ptr<ConfigProducts>> g_cp;
+
+static std::shared_ptr<ConfigProducts> config_products(const tomo_sim_config& c) {
+    tomo_sim_config k = c;
+    k.n_params = 0;
+    k.n_exc = 0;  // per-excitation quantities are not cached
+    const std::string key((const char*)&k, sizeof(k));
+    std::lock_guard<std::mutex> g(g_cp_mu);
+    auto it = g_cp.find(key);
+    if (it != g_cp.end()) return it->second;
+    if (g_cp.size() > 256) g_cp.clear();
+    auto p = std::make_shared<ConfigProducts>();
+    g_cp[key] = p;
+    return p;
+}
+
 static double phys_source_value(const tomo_sim_config& c, int n) {
     const double t = (n + 1) * c.dt;
     const double u = (t - c.pulse_t0) / c.pulse_tau;
@@ -446,7 +478,22 @@ int build_setup(const tomo_sim_config& c, const double* p, size_t np, int exc, S
     }
     if (ref) {
         if ((rc = ref_coefs(c, p, np, s))) return rc;
-        if ((rc = ref_raster(c, s))) return rc;
+        auto cp = config_products(c);
+        {
+            std::lock_guard<std::mutex> g(g_cp_mu);
+            if (!cp->has_ref) {
+                SimSetup tmp;
+                tmp.cfg = c;
+                if ((rc = ref_raster(c, tmp))) return rc;
+                cp->ref = tmp.ref;
+                cp->span = tmp.span;
+                cp->jp_len = tmp.jp_len;
+                cp->has_ref = true;
+            }
+            s.ref = cp->ref;
+            s.span = cp->span;
+            s.jp_len = cp->jp_len;
+        }
         ref_cpml(c, s);
         const float chez = (float)((dt / c.dx) / kMU0);
         s.chx = s.chy = chez;
@@ -487,17 +534,29 @@ int build_setup(const tomo_sim_config& c, const double* p, size_t np, int exc, S
         s.sx = c.nx / 2 + (int)std::lround(c.src_radius * std::cos(th) / c.dx);
         s.sy = c.ny / 2 + (int)std::lround(c.src_radius * std::sin(th) / c.dy);
         if (s.sx < 1 || s.sx >= c.nx || s.sy < 1 || s.sy >= c.ny) return fail(TOMO_E_INVALID, "source outside grid");
-        s.srcw.resize(c.steps);
-        for (int n = 0; n < c.steps; ++n) s.srcw[n] = (float)phys_source_value(c, n);
+        auto cp = config_products(c);
+        std::lock_guard<std::mutex> g(g_cp_mu);
+        if (!cp->has_phys) {
+            cp->srcw.resize(c.steps);
+            for (int n = 0; n < c.steps; ++n) cp->srcw[n] = (float)phys_source_value(c, n);
+            // S(f) = sum_n s(n) dt exp(-j w (n+1) dt), in FP64 with an exact reseed every 256 steps
+            for (int f = 0; f < c.n_freq; ++f) {
+                const double w = 2.0 * kPI * s.freq[f];
+                const std::complex<double> rot = std::exp(std::complex<double>(0.0, -w * dt));
+                std::complex<double> S = 0, z = 0;
+                for (int n = 0; n < c.steps; ++n) {
+                    if ((n & 255) == 0) z = std::exp(std::complex<double>(0.0, -w * (n + 1) * dt));
+                    S += phys_source_value(c, n) * dt * z;
+                    z *= rot;
+                }
+                cp->inv_norm[f] = 1.0 / (8 * kPI * eta0() * std::norm(S));
+            }
+            cp->has_phys = true;
+        }
+        s.srcw = cp->srcw;
+        for (int f = 0; f < c.n_freq; ++f) s.inv_norm[f] = cp->inv_norm[f];
         s.tJ = 0.5;
         s.tM = 1.0;
-        for (int f = 0; f < c.n_freq; ++f) {
-            const double w = 2.0 * kPI * s.freq[f];
-            std::complex<double> S = 0;
-            for (int n = 0; n < c.steps; ++n)
-                S += phys_source_value(c, n) * dt * std::exp(std::complex<double>(0.0, -w * (n + 1) * dt));
-            s.inv_norm[f] = 1.0 / (8 * kPI * eta0() * std::norm(S));
-        }
     }
     return TOMO_OK;
 }

@@ -34,6 +34,14 @@ struct DevSim {
     float* phx[2];
     float* pey[2];
     float* phy[2];
+    // the same buffers as (base, stride): field planes ez0 ez1 hx0 hx1 hy0 hy1
+    // at fbase + k*ps, jp0 jp1 at jbase + k*js, pex0 pex1 phx0 phx1 at
+    // xbase + k*xs, pey0 pey1 phy0 phy1 at ybase + k*ys (strides in floats)
+    float* fbase;
+    float* jbase;
+    float* xbase;
+    float* ybase;
+    int ps, js, xs, ys;
     const uint8_t* mat;
     const int* span;       // lo, hi, off per row
     float* samp;           // 3 x K x P
@@ -72,13 +80,19 @@ struct LaunchItem {
     int nsteps;    // steps to do in this epoch (<= chunk)
 };
 
-// Work unit of the time-blocked update: `waves` adjacent column bands of one
-// row segment of one simulation.
+// Work unit of the time-blocked update: `nband` (<= waves) adjacent column
+// bands over output rows [y0, y1) of one simulation.  Band k owns columns
+// [x0 + k V, min(x0 + (k+1) V, xe)) with V = 64 - 2T and reads T ghost
+// columns on each side.  Interior tiles (no CPML strip, NTFF contour
+// row/column or grid edge within their ghost zone, full-width bands) run the
+// lean interior kernel; the rest the edge kernel.
 struct TileItem {
     int item;      // index into the LaunchItem array
-    int band0;     // first band of the group
-    int seg;       // row segment
-    int pad;
+    int x0;        // first owned column of band 0
+    int nband;     // bands in the group
+    int y0, y1;    // output rows
+    int xe;        // end of the owned columns of the tile (exclusive)
+    int pad[2];
 };
 
 // Projection work description per (simulation, frequency).
