@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the 2D TMz FDTD fitness evaluation on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--autotune] [--serve N]
 
 One "step" is one fitness evaluation of the plain path on the 1024^2,
 dx = 0.25 mm, 8192-step workload (one antenna; one simulation per launch
@@ -9,9 +9,13 @@ sequence), with every input already resident on the device.  `value` is
 Mcell-steps/s of the timed steps (cell-step = one Yee cell advanced one full
 E+H step).  The same JSON line carries:
   * batched: one batch of the 512^2 x 4096-step workload, 64 parameter
-    vectors x 16 antennas = 1024 simulations advanced together (64 evals);
-  * roofline: the time-blocked update kernel's algorithmic bytes per launch
-    over its average launch time, timed with HIP events on the library stream;
+    vectors x 16 antennas = 1024 simulations advanced together (64 evals),
+    and the plain path on the first vectors of the same batch (evals/s ratio);
+  * roofline: the dominant update kernel's algorithmic bytes per launch over
+    its average launch time, timed with HIP events on the stream it runs on,
+    the other update kernels beside it, and the whole update phase; `traffic`
+    is the rocprofv3-measured HBM bytes per launch when recorded for the
+    same schedule (profiles/update_traffic.json);
   * cpu_baseline: the oracle's FP32 restatement on one host core, on a bounded
     sample of the same 1024^2 workload (a baseline, not the target);
   * serving (with --serve): p50/p99 latency of a continuous-batching run.
@@ -51,55 +55,80 @@ def cell_steps(cfg):
     return float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
 
 
-def _kernel_roofline(name, launches, ms, cells, alg_bytes):
-    if launches == 0 or ms <= 0:
+UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner")
+
+
+def _kernel_roofline(name, k):
+    if k["launches"] == 0 or k["ms"] <= 0:
         return None
-    avg_s = ms / launches / 1e3
-    bpl = alg_bytes / launches
+    avg_s = k["ms"] / k["launches"] / 1e3
+    bpl = k["alg_bytes"] / k["launches"]
     achieved = bpl / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-            "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bpl,
-            "alg_bytes_per_cell_step": round(alg_bytes / cells, 3) if cells else None,
-            "cell_steps_per_launch": cells / launches, "total_ms": round(ms, 2)}
+            "launches": k["launches"], "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bpl,
+            "alg_bytes_per_cell_step": round(k["alg_bytes"] / k["cell_steps"], 3) if k["cell_steps"] else None,
+            "cell_steps_per_launch": k["cell_steps"] / k["launches"], "total_ms": round(k["ms"], 2)}
 
 
 def roofline_from(stats):
-    """Roofline of the dominant update kernel (the other one is listed beside it)."""
-    inner = _kernel_roofline("k_tinner", stats["inner_launches"], stats["inner_ms"], stats["inner_cell_steps"],
-                             stats["inner_alg_bytes"])
-    edge = _kernel_roofline("k_tblock", stats["tblock_launches"], stats["tblock_ms"], stats["cell_steps"],
-                            stats["alg_bytes"])
-    ks = [k for k in (inner, edge) if k]
+    """Roofline of the dominant update kernel (the one that moves the most
+    algorithmic bytes; the edge kernels run beside it on their own streams),
+    the other update kernels, and the whole update phase (every stream, wall
+    time) as one effective rate."""
+    ks = [r for r in (_kernel_roofline(n, stats[n]) for n in UPDATE_KERNELS) if r]
     if not ks:
         return None
-    dom = max(ks, key=lambda k: k["total_ms"])
-    dom = dict(dom)
-    dom["other_kernels"] = [k for k in ks if k["kernel"] != dom["kernel"]]
+    dom = dict(max(ks, key=lambda r: r["alg_bytes_per_launch"] * r["launches"]))
+    dom["other_kernels"] = [r for r in ks if r["kernel"] != dom["kernel"]]
+    u = stats["update"]
+    if u["ms"] > 0:
+        eff = u["alg_bytes"] / (u["ms"] / 1e3) / 1e9
+        dom["update_phase"] = {"achieved": round(eff, 1), "frac": round(eff / HBM_PEAK_GBS, 4),
+                               "wall_ms": round(u["ms"], 2), "epochs": u["launches"],
+                               "Mcell_steps_per_s": round(u["cell_steps"] / (u["ms"] / 1e3) / 1e6, 1)}
     return dom
 
 
-def traffic_for(workload, schedule):
-    """Measured HBM bytes per update launch (profiles/k_tblock_traffic.json), if recorded."""
-    path = os.path.join(ROOT, "profiles", "k_tblock_traffic.json")
+def traffic_for(workload, kernel, schedule):
+    """Measured HBM bytes per launch of `kernel` (profiles/update_traffic.json:
+    rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), if recorded for this workload and schedule."""
+    path = os.path.join(ROOT, "profiles", "update_traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        rec = json.load(open(path)).get(workload)
+        rec = json.load(open(path)).get(workload, {})
     except (OSError, ValueError):
         return None
-    if rec and tuple(rec.get("schedule", ())) == tuple(schedule):
-        return rec.get("hbm_bytes_per_launch")
-    return None
+    if tuple(rec.get("schedule", ())) != tuple(schedule):
+        return None
+    return rec.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+
+
+def kernel_ms(st, per=1.0):
+    return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
 
 
 SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult")
+# Default schedules (picked by Context.autotune on MI355X; fixed so that the
+# profiled command and the timed one run the same launches).
+PLAIN_SCHEDULE = dict(tsteps=8, seg_rows=8, edge_waves=2, inner_waves=4, chunk=32, grid_mult=2)
+BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=8)
 
 
-def run_plain(ctx, steps, warmup, tune=True):
-    cfg = t.preset(t.PRESET_CFG2)
+def apply_schedule(ctx, cfg, n_sims, sched, tune):
     if tune:
-        ctx.autotune(cfg, 1)
+        ctx.autotune(cfg, n_sims)
+    else:
+        ctx.set_schedule(**sched)
+    return dict(zip(SCHED_KEYS, ctx.schedule()))
+
+
+def run_plain(ctx, steps, warmup, sched, tune):
+    """cfg2: one 1024^2 simulation per eval, one eval after another."""
+    cfg = t.preset(t.PRESET_CFG2)
+    used = apply_schedule(ctx, cfg, 1, sched, tune)
     vecs = seeded_vectors(steps + warmup, 8, 20261202)
     for i in range(warmup):
         ctx.eval(cfg, vecs[i])  # the first call also builds and caches the target far field
@@ -110,14 +139,18 @@ def run_plain(ctx, steps, warmup, tune=True):
     st = ctx.stats()
     ctx.profile(False)
     assert all(np.isfinite(fits))
-    return cfg, dt, st
+    return cfg, dt, st, used
 
 
-def run_batched(ctx, tune=True):
+def batch_vectors():
+    return seeded_vectors(64, 8, 20261203)
+
+
+def run_batched(ctx, sched, tune):
+    """cfg3: 64 vectors x 16 antennas = 1024 simulations advanced together."""
     cfg = t.preset(t.PRESET_CFG3)
-    if tune:
-        ctx.autotune(cfg, 128)
-    vecs = seeded_vectors(64, 8, 20261203)
+    used = apply_schedule(ctx, cfg, 128, sched, tune)
+    vecs = batch_vectors()
     ctx.target_farfield(cfg)  # warm-up: the 16-antenna target phantom
     ctx.profile(True)
     t0 = time.perf_counter()
@@ -126,7 +159,22 @@ def run_batched(ctx, tune=True):
     st = ctx.stats()
     ctx.profile(False)
     assert np.all(np.isfinite(fits))
-    return cfg, dt, st, len(vecs)
+    return cfg, dt, st, len(vecs), fits, used
+
+
+def run_plain_same_workload(ctx, n, sched, batch_fits):
+    """The plain path on the batched workload: the first n vectors of the cfg3
+    batch, each evaluated by tomo_eval (its 16 simulations one at a time)."""
+    cfg = t.preset(t.PRESET_CFG3)
+    ctx.set_schedule(**sched)
+    vecs = batch_vectors()[:n]
+    ctx.eval(cfg, vecs[0])  # warm-up (tile lists, caches)
+    t0 = time.perf_counter()
+    fits = [ctx.eval(cfg, v) for v in vecs]
+    dt = time.perf_counter() - t0
+    rel = max(abs(f - b) / abs(b) for f, b in zip(fits, batch_fits[:n]))
+    return {"evals": n, "s": round(dt, 3), "evals_per_s": round(n / dt, 3),
+            "Mcell_steps_per_s": round(cell_steps(cfg) * n / dt / 1e6, 1), "max_rel_fitness_diff_vs_batched": rel}
 
 
 def run_serve(ctx, n_req, rate, seed=20261205):
@@ -221,7 +269,11 @@ def main():
     ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--schedule", type=str, default="",
-                    help="tsteps,seg_rows,edge_waves,chunk[,inner_waves,grid_mult]: fixed schedule, no autotuning")
+                    help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult")
+    ap.add_argument("--batch-schedule", type=str, default="", help="batched-path schedule (same fields)")
+    ap.add_argument("--autotune", action="store_true", help="pick both schedules with the autotuner instead")
+    ap.add_argument("--plain-evals", type=int, default=2,
+                    help="plain-path evals on the batched workload (for the batched/plain ratio; 0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -229,11 +281,11 @@ def main():
               "(simulations are independent: more GPUs would each run their own share).", file=sys.stderr)
         return 2
     ctx = t.Context()
-    tune = not args.schedule
+    tune = args.autotune
+    psched = dict(PLAIN_SCHEDULE)
     if args.schedule:
-        ctx.set_schedule(*[int(x) for x in args.schedule.split(",")])
-    cfg, dt, st = run_plain(ctx, args.steps, args.warmup, tune)
-    sched = ctx.schedule()
+        psched = dict(zip(SCHED_KEYS, [int(x) for x in args.schedule.split(",")]))
+    cfg, dt, st, sched = run_plain(ctx, args.steps, args.warmup, psched, tune)
     cs = cell_steps(cfg) * args.steps
     value = cs / dt / 1e6
     out = {"metric": METRIC, "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -242,33 +294,32 @@ def main():
            "data": "synthetic (seeded phantoms)",
            "config": {"workload": "plain: 1024^2 cells, dx 0.25 mm, 8192 steps, 1 antenna, 8 freq x 180 angles, "
                                   "one simulation per launch sequence",
-                      "preset": "CFG2", "schedule": dict(zip(SCHED_KEYS, sched)),
+                      "preset": "CFG2", "schedule": sched,
                       "schedule_source": "autotuned" if tune else "fixed"},
            "evals_per_s": round(args.steps / dt, 3)}
     rl = roofline_from(st)
     if rl:
-        rl["traffic"] = traffic_for("plain_cfg2", sched)
+        rl["traffic"] = traffic_for("plain_cfg2", rl["kernel"], tuple(sched.values()))
     out["roofline"] = rl
-    out["kernel_ms_per_step"] = {"k_tinner": round(st["inner_ms"] / args.steps, 3),
-                                 "k_tblock": round(st["tblock_ms"] / args.steps, 3),
-                                 "k_fold": round(st["fold_ms"] / args.steps, 3),
-                                 "k_project": round(st["project_ms"] / args.steps, 3)}
+    out["kernel_ms_per_step"] = kernel_ms(st, args.steps)
     if not args.no_batched:
-        bcfg, bdt, bst, nvec = run_batched(ctx, tune)
-        bsched = ctx.schedule()
+        bsch = dict(BATCH_SCHEDULE)
+        if args.batch_schedule:
+            bsch = dict(zip(SCHED_KEYS, [int(x) for x in args.batch_schedule.split(",")]))
+        bcfg, bdt, bst, nvec, bfits, bsched = run_batched(ctx, bsch, tune)
         bcs = cell_steps(bcfg) * nvec
         brl = roofline_from(bst)
         if brl:
-            brl["traffic"] = traffic_for("batched_cfg3", bsched)
+            brl["traffic"] = traffic_for("batched_cfg3", brl["kernel"], tuple(bsched.values()))
         out["batched"] = {"workload": "512^2, dx 0.5 mm, 4096 steps; 64 vectors x 16 antennas = 1024 simulations, "
                                       "16 freq x 360 angles (CFG3)",
                           "Mcell_steps_per_s": round(bcs / bdt / 1e6, 1), "evals_per_s": round(nvec / bdt, 3),
-                          "s_per_batch": round(bdt, 3), "roofline": brl,
-                          "schedule": dict(zip(SCHED_KEYS, bsched)),
-                          "kernel_ms": {"k_tinner": round(bst["inner_ms"], 1), "k_tblock": round(bst["tblock_ms"], 1),
-                                        "k_fold": round(bst["fold_ms"], 1),
-                                        "k_project": round(bst["project_ms"], 1)}}
-        out["batched_vs_plain_cell_rate"] = round(out["batched"]["Mcell_steps_per_s"] / value, 3)
+                          "s_per_batch": round(bdt, 3), "roofline": brl, "schedule": bsched,
+                          "kernel_ms": kernel_ms(bst)}
+        if args.plain_evals:
+            pl = run_plain_same_workload(ctx, args.plain_evals, psched if not tune else dict(zip(SCHED_KEYS, ctx.schedule())), bfits)
+            out["batched"]["plain_same_workload"] = pl
+            out["batched"]["batched_vs_plain_evals_per_s"] = round(out["batched"]["evals_per_s"] / pl["evals_per_s"], 3)
     if args.serve:
         out["serving"] = run_serve(ctx, args.serve, args.rate)
     if args.calib_copy_mb:

@@ -144,24 +144,33 @@ int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
 int tomo_autotune(tomo_ctx *ctx, const tomo_sim_config *cfg, int n_sims, tomo_schedule *best,
                   double *cell_steps_per_s);
 
-/* Kernel timing: when enabled, every launch of the time-blocked update, the
- * running-DFT fold and the projection is bracketed by HIP events on the
- * library's stream.  alg_bytes counts the algorithmic HBM bytes of the update
- * launches: cell-steps x (24 + 1 + 8 f_disp + 16 f_strip), where f_disp and
- * f_strip are each simulation's fractions of Debye and CPML-strip cells. */
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
+ * on the stream it is launched on, and its duration is credited to its
+ * kernel.  The update kernels also count the cell-steps they advance and
+ * their algorithmic HBM bytes: per cell-step 24 (Ez, Hx, Hy read and written
+ * once) + 1 (material id) + 8 on Debye cells (Jp) + 16 on CPML-strip cells
+ * (the psi pair).  `update` is the wall time of the whole update phase (all
+ * streams, from the first launch of an epoch to the join of its last block). */
+enum {
+    TOMO_KERNEL_GENERAL = 0, /* k_tblock: every tile of a reference-compatible simulation */
+    TOMO_KERNEL_CORNER = 1,  /* k_tedge: corner tiles (and all tiles of small physical grids) */
+    TOMO_KERNEL_ROW = 2,     /* k_trow: top / bottom frame between the corners */
+    TOMO_KERNEL_COL = 3,     /* k_tcol: left / right frame between the corners */
+    TOMO_KERNEL_INNER = 4,   /* k_tinner: clean interior bands */
+    TOMO_KERNEL_FOLD = 5,    /* k_fold: running-DFT fold of a chunk of samples */
+    TOMO_KERNEL_PROJECT = 6, /* k_project: far-field projection */
+    TOMO_KERNEL_FITNESS = 7, /* k_fitness: FP64 fitness reduction */
+    TOMO_KERNEL_COUNT = 8
+};
+typedef struct tomo_kernel_stats {
+    int64_t launches;
+    double ms;         /* summed launch durations */
+    double cell_steps; /* update kernels only */
+    double alg_bytes;  /* update kernels only */
+} tomo_kernel_stats;
 typedef struct tomo_stats {
-    int64_t tblock_launches; /* general (edge-tile) update kernel */
-    double tblock_ms;
-    double cell_steps;
-    double alg_bytes;
-    int64_t inner_launches;  /* interior-tile update kernel */
-    double inner_ms;
-    double inner_cell_steps;
-    double inner_alg_bytes;
-    int64_t fold_launches;
-    double fold_ms;
-    int64_t project_launches;
-    double project_ms;
+    tomo_kernel_stats kernel[TOMO_KERNEL_COUNT];
+    tomo_kernel_stats update; /* launches = epochs */
 } tomo_stats;
 int tomo_profile(tomo_ctx *ctx, int enable); /* (re)starts the counters */
 int tomo_get_stats(tomo_ctx *ctx, tomo_stats *out);

@@ -234,3 +234,27 @@ def test_autotune_installs_a_schedule_that_keeps_results(oracle):
     assert rel_l2(g.fields()[0], o.fields()[0]) < FIELD_TOL_F32
     g.close()
     c.close()
+
+
+def test_kernel_stats_account_for_every_cell_step():
+    """Per-kernel timing: the update kernels' cell-steps add up to the
+    simulations' cell-steps, every tile class of a split grid runs, and the
+    algorithmic bytes per cell-step stay within the 25..49 bytes the formula
+    allows."""
+    cfg = phys_small(steps=96, n=256, n_exc=2)
+    c = t.Context()
+    c.set_schedule(4, 16, 2, 32, 2, 2)
+    c.target_farfield(cfg)  # the target's own simulations stay out of the counters
+    c.profile(True)
+    c.eval(cfg, PHYS_PARAMS)
+    st = c.stats()
+    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner")]
+    want = float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
+    assert sum(k["cell_steps"] for k in upd) == want
+    assert st["update"]["cell_steps"] == want
+    for k in ("k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness"):
+        assert st[k]["launches"] > 0 and st[k]["ms"] > 0, k
+    assert st["k_tblock"]["launches"] == 0
+    b = sum(k["alg_bytes"] for k in upd) / want
+    assert 25.0 < b < 49.0
+    c.close()

@@ -38,17 +38,29 @@ class Schedule(_c.Structure):
         return tuple(getattr(self, k) for k, _ in self._fields_)
 
 
-class Stats(_c.Structure):
-    """tomo_stats (include/tomo.h)."""
+KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness")
 
-    _fields_ = [("tblock_launches", _c.c_int64), ("tblock_ms", _c.c_double), ("cell_steps", _c.c_double),
-                ("alg_bytes", _c.c_double), ("inner_launches", _c.c_int64), ("inner_ms", _c.c_double),
-                ("inner_cell_steps", _c.c_double), ("inner_alg_bytes", _c.c_double),
-                ("fold_launches", _c.c_int64), ("fold_ms", _c.c_double),
-                ("project_launches", _c.c_int64), ("project_ms", _c.c_double)]
+
+class KernelStats(_c.Structure):
+    """tomo_kernel_stats (include/tomo.h)."""
+
+    _fields_ = [("launches", _c.c_int64), ("ms", _c.c_double), ("cell_steps", _c.c_double),
+                ("alg_bytes", _c.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(_c.Structure):
+    """tomo_stats (include/tomo.h): one record per kernel (TOMO_KERNEL_*) and
+    one for the whole update phase."""
+
+    _fields_ = [("kernel", KernelStats * len(KERNEL_NAMES)), ("update", KernelStats)]
+
+    def as_dict(self):
+        d = {name: self.kernel[i].as_dict() for i, name in enumerate(KERNEL_NAMES)}
+        d["update"] = self.update.as_dict()
+        return d
 
 
 class TomoError(RuntimeError):

@@ -54,8 +54,10 @@ struct Slot {
     long req = -1;      // owning request (serving)
     int exc = 0;
     uint64_t serial = 0; // unique per simulation (table indices are reused)
-    // cell accounting of the current tile split (for kernel timing)
-    double inner_nodes = 0, inner_jp = 0;
+    // cell accounting of the current tile split (for kernel timing): cells and
+    // algorithmic bytes per step of the corner, row-edge, column-edge and
+    // interior tile classes
+    double cls_cells[4] = {0, 0, 0, 0}, cls_bytes[4] = {0, 0, 0, 0};
     std::vector<float> hpos;  // host copy of the contour positions (source of an async upload)
 };
 
@@ -138,7 +140,7 @@ struct tomo_ctx {
     bool prof = false;
     tomo_stats stats{};
     std::vector<hipEvent_t> ev_pool;
-    struct Pending { hipEvent_t a, b; int kind; double cells, bytes; };
+    struct Pending { hipEvent_t a, b; int kind; double cells, bytes; };  // kind: TOMO_KERNEL_*, or -1 = update
     std::vector<Pending> pending;
 };
 
@@ -372,11 +374,26 @@ int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); 
 void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<TileItem>& edge,
                  std::vector<TileItem>& rowedge, std::vector<TileItem>& coledge, std::vector<TileItem>& inner) {
     const DevSim& d = sl->d;
-    sl->inner_nodes = 0;
-    sl->inner_jp = 0;
+    for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
     const int nx = d.nx, ny = d.ny, nc = d.nc, D = d.D, V = 64 - 2 * T;
+    // Algorithmic bytes of the cells a tile owns inside [0, nx) x [0, ny):
+    // Ez, Hx, Hy read and written once (24) and the material id (1) on every
+    // cell, Jp (8) on Debye cells, the psi pair (16) on CPML-strip cells.
+    auto account = [&](int k, const TileItem& t) {
+        const int x0 = std::max(t.x0, 0), x1 = std::min(t.xe, nx);
+        const int y0 = std::max(t.y0, 0), y1 = std::min(t.y1, ny);
+        if (x1 <= x0 || y1 <= y0) return;
+        const double cells = (double)(x1 - x0) * (y1 - y0);
+        double jp = 0;
+        for (int y = y0; y < y1; ++y)
+            jp += std::max(0, std::min(x1, sl->su.span[3 * y + 1]) - std::max(x0, sl->su.span[3 * y]));
+        const double cx = std::max(0, std::min(x1, nx - nc) - std::max(x0, nc));
+        const double cy = std::max(0, std::min(y1, ny - nc) - std::max(y0, nc));
+        sl->cls_cells[k] += cells;
+        sl->cls_bytes[k] += 25.0 * cells + 8.0 * jp + 16.0 * (cells - cx * cy);
+    };
     // column ranges [a, b) split into bands of V and grouped w bands per tile
-    auto add = [&](std::vector<TileItem>& v, int a, int b, int y0, int y1, int w) {
+    auto add = [&](int k, std::vector<TileItem>& v, int a, int b, int y0, int y1, int w) {
         for (int x = a; x < b; x += w * V) {
             TileItem t{};
             t.item = item;
@@ -386,6 +403,7 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<Ti
             t.y0 = y0;
             t.y1 = y1;
             v.push_back(t);
+            account(k, t);
         }
     };
     // clean rows / columns lie strictly between the special ones
@@ -396,11 +414,8 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<Ti
     const int ixr = ixl + nbi * V;                    // end of interior owned columns
     const int ia = ybot + T, ib = ytop - T + 1;       // interior output rows [ia, ib)
     const bool split = d.mode == TOMO_MODE_PHYSICAL && nbi > 0 && ib > ia;
-    auto edge_rows = [&](int y0, int y1) {
-        for (int a = y0; a < y1; a += S) add(edge, 0, nx + 1, a, std::min(a + S, y1), W);
-    };
     if (!split) {
-        edge_rows(0, ny + 1);
+        for (int a = 0; a < ny + 1; a += S) add(0, edge, 0, nx + 1, a, std::min(a + S, ny + 1), W);
         return;
     }
     // top and bottom rows: corner bands to the general kernel, clean-column
@@ -409,35 +424,19 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<Ti
         const int sc = std::min(S, 8);  // corner tiles: short row chunks (their code path is the slowest)
         for (int a = y0; a < y1; a += sc) {
             const int a1 = std::min(a + sc, y1);
-            add(edge, 0, ixl, a, a1, W);
-            add(edge, ixr, nx + 1, a, a1, W);
+            add(0, edge, 0, ixl, a, a1, W);
+            add(0, edge, ixr, nx + 1, a, a1, W);
         }
-        for (int a = y0; a < y1; a += S) add(rowedge, ixl, ixr, a, std::min(a + S, y1), Wi);
+        for (int a = y0; a < y1; a += S) add(1, rowedge, ixl, ixr, a, std::min(a + S, y1), Wi);
     };
     edge_rows_split(0, ia);
     for (int a = ia; a < ib; a += S) {
         const int a1 = std::min(a + S, ib);
-        add(coledge, 0, ixl, a, a1, W);
-        add(inner, ixl, ixr, a, a1, Wi);
-        add(coledge, ixr, nx + 1, a, a1, W);
-        for (int y = a; y < a1; ++y) {
-            const int lo = std::max(ixl, sl->su.span[3 * y]), hi = std::min(ixr, sl->su.span[3 * y + 1]);
-            sl->inner_jp += std::max(0, hi - lo);
-        }
-        sl->inner_nodes += (double)(ixr - ixl) * (a1 - a);
+        add(2, coledge, 0, ixl, a, a1, W);
+        add(3, inner, ixl, ixr, a, a1, Wi);
+        add(2, coledge, ixr, nx + 1, a, a1, W);
     }
     edge_rows_split(ib, ny + 1);
-}
-
-// Algorithmic HBM bytes per cell-step of one simulation: Ez, Hx, Hy read and
-// written once (24), the material id (1), Jp on Debye cells (8 f_disp) and the
-// psi pair on CPML-strip cells (16 f_strip).
-double alg_bytes_per_cell_step(const Slot* s) {
-    const double nx = s->d.nx, ny = s->d.ny, nc = s->d.nc;
-    const double cells = nx * ny;
-    const double f_disp = (double)s->su.jp_len / cells;
-    const double f_strip = (2 * nc * ny + 2 * nc * nx - 4 * nc * nc) / cells;
-    return 24.0 + 1.0 + 8.0 * f_disp + 16.0 * f_strip;
 }
 
 hipEvent_t pool_event(tomo_ctx* c) {
@@ -451,7 +450,7 @@ hipEvent_t pool_event(tomo_ctx* c) {
     return e;
 }
 
-// Resolve recorded launch timings (synchronises the stream).
+// Resolve recorded launch timings (synchronises the streams).
 int drain_timings(tomo_ctx* c) {
     if (c->pending.empty()) return TOMO_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -459,25 +458,11 @@ int drain_timings(tomo_ctx* c) {
     for (auto& p : c->pending) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
-        if (p.kind == 0) {
-            c->stats.tblock_launches++;
-            c->stats.tblock_ms += ms;
-            c->stats.cell_steps += p.cells;
-            c->stats.alg_bytes += p.bytes;
-        } else if (p.kind == 4) {
-            c->stats.tblock_ms += ms;
-        } else if (p.kind == 3) {
-            c->stats.inner_launches++;
-            c->stats.inner_ms += ms;
-            c->stats.inner_cell_steps += p.cells;
-            c->stats.inner_alg_bytes += p.bytes;
-        } else if (p.kind == 1) {
-            c->stats.fold_launches++;
-            c->stats.fold_ms += ms;
-        } else {
-            c->stats.project_launches++;
-            c->stats.project_ms += ms;
-        }
+        tomo_kernel_stats& k = p.kind < 0 ? c->stats.update : c->stats.kernel[p.kind];
+        k.launches++;
+        k.ms += ms;
+        k.cell_steps += p.cells;
+        k.alg_bytes += p.bytes;
         c->ev_pool.push_back(p.a);
         c->ev_pool.push_back(p.b);
     }
@@ -569,28 +554,32 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
             const bool multi = mode == TOMO_MODE_PHYSICAL && n_inner > 0 && nside > 0;
+            // per-class cell-steps and bytes of block o (kernel timing only)
+            auto block_work = [&](int o, int k, double& cells, double& bytes) {
+                cells = bytes = 0;
+                if (!c->prof) return;
+                for (size_t i = 0; i < ms.size(); ++i) {
+                    const int nst = std::min(T, c->h_items[i].nsteps - o * T);
+                    if (nst <= 0) continue;
+                    cells += nst * ms[i]->cls_cells[k];
+                    bytes += nst * ms[i]->cls_bytes[k];
+                }
+            };
+            LaunchTimer whole(c, c->stream);
             if (multi) HIPCHK(hipEventRecord(c->ev_bar, c->stream));
+            double ucells = 0, ubytes = 0;
             for (int o = 0; o < nl; ++o) {
-                double ecells = 0, ebytes = 0, icells = 0, ibytes = 0;
-                if (c->prof) {
-                    for (size_t i = 0; i < ms.size(); ++i) {
-                        const int nst = std::min(T, c->h_items[i].nsteps - o * T);
-                        if (nst <= 0) continue;
-                        const Slot* q = ms[i];
-                        const double all_c = (double)q->d.nx * q->d.ny;
-                        const double all_b = all_c * alg_bytes_per_cell_step(q);
-                        const double in_b = q->inner_nodes * 25.0 + q->inner_jp * 8.0;
-                        icells += nst * q->inner_nodes;
-                        ibytes += nst * in_b;
-                        ecells += nst * (all_c - q->inner_nodes);
-                        ebytes += nst * (all_b - in_b);
-                    }
+                double cells[4], bytes[4];
+                for (int k = 0; k < 4; ++k) {
+                    block_work(o, k, cells[k], bytes[k]);
+                    ucells += cells[k];
+                    ubytes += bytes[k];
                 }
                 if (mode != TOMO_MODE_PHYSICAL) {
                     LaunchTimer lt(c, c->stream);
                     HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
                                          std::min(n_edge, cap), c->stream));
-                    lt.done(0, ecells, ebytes);
+                    lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
                     continue;
                 }
                 // every class of block o starts after all of block o-1 (ev_bar)
@@ -606,15 +595,14 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     if (k == 0) HIPCHK(launch_tedge(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
                     if (k == 1) HIPCHK(launch_trow(T, tl, n, c->d_items, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
                     if (k == 2) HIPCHK(launch_tcol(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
-                    // the three edge launches of a block share one timing record (kind 0)
-                    lt.done(k == 0 ? 0 : 4, k == 0 ? ecells : 0, k == 0 ? ebytes : 0);
+                    lt.done(TOMO_KERNEL_CORNER + k, cells[k], bytes[k]);
                     if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
                 }
                 if (n_inner > 0) {
                     LaunchTimer lt(c, c->stream);
                     HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, c->d_items, c->d_sims, o, c->Wi,
                                          std::min(n_inner, 2 * cap), c->stream));
-                    lt.done(3, icells, ibytes);
+                    lt.done(TOMO_KERNEL_INNER, cells[3], bytes[3]);
                 }
                 if (multi) {
                     for (int k = 0; k < 3; ++k)
@@ -622,11 +610,12 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     HIPCHK(hipEventRecord(c->ev_bar, c->stream));
                 }
             }
+            whole.done(-1, ucells, ubytes);
             {
                 LaunchTimer lt(c);
                 if (fp32_any) HIPCHK(launch_fold(false, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
                 if (fp64_any) HIPCHK(launch_fold(true, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
-                lt.done(1);
+                lt.done(TOMO_KERNEL_FOLD);
             }
             if (c->pending.size() > 4096) {
                 int rc2 = drain_timings(c);
@@ -687,7 +676,7 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
     HIPCHK(hipMemcpyAsync(d, items.data(), sizeof(ProjItem) * items.size(), hipMemcpyHostToDevice, c->stream));
     LaunchTimer lt(c);
     hipError_t e = launch_project(d, (int)items.size(), c->d_sims, nullptr, A, (float)eta0(), dD, dP, c->stream);
-    lt.done(2);
+    lt.done(TOMO_KERNEL_PROJECT);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipFree(d));
     HIPCHK(e);
@@ -779,8 +768,10 @@ static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* para
     }
     if (!rc) {
         const int terms = (tph ? 1 : 0) | ((cfg.flags & TOMO_FLAG_REL_FITNESS) ? 2 : 0);
+        LaunchTimer lt(c);
         hipError_t e = launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, (int)n_vec, (double)E * F,
                                       terms, dF, c->stream);
+        lt.done(TOMO_KERNEL_FITNESS);
         if (e != hipSuccess) rc = fail(TOMO_E_HIP, hipGetErrorString(e));
     }
     if (!rc) {

@@ -1,0 +1,112 @@
+"""Reduce rocprofv3 CSV output to the per-kernel figures the library reports.
+
+    python -m tomography_amd.rocprof_summary stats DIR PREFIX
+        per-kernel calls / total / average duration from PREFIX_kernel_stats.csv
+    python -m tomography_amd.rocprof_summary pmc DIR PREFIX COUNTER
+        per-kernel, per-template average of COUNTER (KiB per dispatch) from
+        PREFIX_counter_collection.csv, plus the copy-kernel dispatches used
+        for calibration
+
+Kernel names are shortened to `k_name<T>` (the fused step count) so the two
+bench workloads, which run different T, stay apart.
+"""
+import csv
+import json
+import os
+import re
+import sys
+from collections import defaultdict
+
+_NAME = re.compile(r"(k_[a-z0-9_]+)(?:<([^>]*)>)?")
+
+
+def short_name(full):
+    """'void tomo::(anonymous namespace)::k_tinner<4>(...)' -> 'k_tinner<4>'."""
+    head = full.split("(")[0] if "::" not in full.split("(")[0] else full
+    for m in _NAME.finditer(head):
+        return m.group(1) + (f"<{m.group(2)}>" if m.group(2) else "")
+    return full.split("(")[0].strip()[:60]
+
+
+def stats(d, prefix):
+    rows = list(csv.DictReader(open(os.path.join(d, f"{prefix}_kernel_stats.csv"))))
+    out = {}
+    for r in rows:
+        n = short_name(r["Name"])
+        out[n] = {"calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) / 1e6,
+                  "avg_us": float(r["AverageNs"]) / 1e3, "min_us": float(r["MinNs"]) / 1e3,
+                  "max_us": float(r["MaxNs"]) / 1e3, "pct": float(r["Percentage"])}
+    return out
+
+
+def _last(spec):
+    out = {}
+    for a in spec:
+        k, n = a.split("=")
+        out[k] = int(n)
+    return out
+
+
+def trace(d, prefix, spec=()):
+    last = _last(spec)
+    per = defaultdict(list)
+    info = {}
+    for r in csv.DictReader(open(os.path.join(d, f"{prefix}_kernel_trace.csv"))):
+        n = short_name(r["Kernel_Name"])
+        per[n].append((int(r["Dispatch_Id"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+        info[n] = {"vgpr": int(r["VGPR_Count"]), "agpr": int(r["Accum_VGPR_Count"]), "sgpr": int(r["SGPR_Count"]),
+                   "lds": int(r["LDS_Block_Size"]), "scratch": int(r["Scratch_Size"]),
+                   "wg": int(r["Workgroup_Size_X"]), "grid": int(r["Grid_Size_X"])}
+    out = {}
+    for n, v in sorted(per.items(), key=lambda kv: -sum(x[1] for x in kv[1])):
+        v.sort()
+        rec = {"calls": len(v), "total_ms": sum(x[1] for x in v) / 1e6, "avg_us": sum(x[1] for x in v) / len(v) / 1e3}
+        if n in last:
+            w = v[-last[n]:]
+            rec["last"] = len(w)
+            rec["avg_us_last"] = sum(x[1] for x in w) / len(w) / 1e3
+        rec.update(info[n])
+        out[n] = rec
+    return out
+
+
+def pmc(d, prefix, counter, spec=()):
+    last = _last(spec)
+    per = defaultdict(list)
+    copies = []
+    for r in csv.DictReader(open(os.path.join(d, f"{prefix}_counter_collection.csv"))):
+        if r["Counter_Name"] != counter:
+            continue
+        v = float(r["Counter_Value"])
+        n = short_name(r["Kernel_Name"])
+        if "copy" in r["Kernel_Name"].lower():
+            copies.append({"kernel": r["Kernel_Name"][:80], "grid": int(r["Grid_Size"]), "kib": v})
+        per[n].append((int(r["Dispatch_Id"]), v))
+    kern = {}
+    for n, v in sorted(per.items()):
+        v.sort()
+        rec = {"dispatches": len(v), "avg_kib": sum(x[1] for x in v) / len(v)}
+        if n in last:
+            w = v[-last[n]:]
+            rec["last"] = len(w)
+            rec["avg_kib_last"] = sum(x[1] for x in w) / len(w)
+        kern[n] = rec
+    return {"counter": counter, "unit": "KiB per dispatch", "kernels": kern,
+            "largest_copy_dispatches": sorted(copies, key=lambda c: -c["kib"])[:3]}
+
+
+def main(argv):
+    if len(argv) >= 3 and argv[0] == "stats":
+        print(json.dumps(stats(argv[1], argv[2]), indent=1))
+    elif len(argv) >= 3 and argv[0] == "trace":
+        print(json.dumps(trace(argv[1], argv[2], argv[3:]), indent=1))
+    elif len(argv) >= 4 and argv[0] == "pmc":
+        print(json.dumps(pmc(argv[1], argv[2], argv[3], argv[4:]), indent=1))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
