@@ -26,6 +26,60 @@ namespace tomo {
 namespace {
 
 constexpr int kWave = 64;
+// Rows per prefetch group (row_stream) of each update kernel.  The interior
+// kernel is light enough to hold four rows in flight; the edge kernels
+// already run near their register budget, and their throughput in large
+// batches depends on occupancy more than on prefetch depth.
+constexpr int kGroupInner = 4, kGroupRow = 1, kGroupCol = 1, kGroupEdge = 1, kGroupRef = 2;
+// s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding).
+constexpr int kWaitVm0 = 0x0F70;
+
+// Streams rows [y0 - T, y1 + T) of one band through a T-stage register
+// pipeline: step(r, row r at level n) returns row r - T at level n + T, which
+// store() writes when it is an owned output row.  Rows are requested in
+// groups of G.  At a group boundary one vmcnt(0) retires this group's loads
+// and the previous boundary's stores, both issued a whole group of compute
+// earlier; then the previous group's results are stored and the next group's
+// rows requested.  (Vector loads and stores share one counter that may retire
+// them out of order, so a wait on any load issued before a store drains the
+// counter: one wait per group instead of one per row keeps G rows in flight.)
+template <int T, int G, class C, class Load, class Store, class Step>
+__device__ __forceinline__ void row_stream(int y0, int y1, bool own, Load& load, Store& store, Step& step) {
+    const int rb = y0 - T, re = y1 + T;
+    C A[G], R[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+        if (rb + i < re) load(rb + i, A[i]);
+    for (int g0 = rb; g0 < re; g0 += G) {
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        C X[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) X[i] = A[i];
+        if (g0 > rb) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const int yo = g0 - G + i - T;
+                if (yo >= y0 && yo < y1 && own) store(yo, R[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+            if (g0 + G + i < re) load(g0 + G + i, A[i]);
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const int r = g0 + i;
+            if (r >= re) break;
+            R[i] = step(r, X[i]);
+        }
+    }
+    // the last group (rows of a partial group at or past re were not computed)
+    const int gl = rb + ((re - rb - 1) / G) * G;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const int yo = gl + i - T;
+        if (gl + i < re && yo >= y0 && yo < y1 && own) store(yo, R[i]);
+    }
+}
 
 // Global-address-space views: loads and stores through these compile to
 // global_* instructions (not flat_*), so LDS waits never wait on HBM.
@@ -34,6 +88,13 @@ typedef GAS float gfloat;
 typedef GAS const float gcfloat;
 typedef GAS const uint8_t gcu8;
 typedef GAS const int gcint;
+// Constant-address-space views of data that is read-only for a whole launch
+// (row spans, the source waveform): wave-uniform loads through these become
+// scalar loads (s_load_*), which are tracked by lgkmcnt and so never force a
+// wait on the vector loads prefetched rows ahead.
+#define CAS __attribute__((address_space(4)))
+typedef CAS const int kcint;
+typedef CAS const float kcfloat;
 
 // Lane shifts on the VALU (DPP wave_shl:1 / wave_shr:1): next(v)[i] = v[i+1],
 // prev(v)[i] = v[i-1].  The edge lane keeps a stale value; it is a ghost lane.
@@ -63,9 +124,9 @@ struct SimView {
     gfloat* xq;    // pex0 pex1 phx0 phx1, stride xs
     gfloat* yq;    // pey0 pey1 phy0 phy1, stride ys
     gcu8* mat;
-    gcint* span;
+    kcint* span;
     gfloat* samp;
-    gcfloat* srcw;
+    kcfloat* srcw;
     int ps, js, xs, ys, par;
     int nx, ny, pitch, nc, D, P, L1, L2, L3, cx_hi, cy_lo, sx, sy, flags, K;
     float chx, chy, chx_d, chy_d, dxf, dyf;
@@ -90,18 +151,22 @@ __device__ __forceinline__ bool row_ystrip_h(const SimView& S, int r) {
 __device__ __forceinline__ void load_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x, bool xin,
                                           const WaveFlags& wf, Cell& c) {
     c = zero_cell();
+    // The material id is read unconditionally from the clamped node: a byte
+    // loaded under a branch is zero-extended right after the branch, which
+    // would force a wait on every load in flight.  Outside the grid the id
+    // only selects coefficients of a cell whose result is discarded.
+    c.mat = S.mat[min(max(r, 0), S.ny) * S.pitch + min(max(x, 0), S.nx)];
     if (r < 0 || r > S.ny) return;  // wave-uniform
     if (!xin) return;
     const int base = r * S.pitch + x;
     const int in_f = S.par * S.ps;
     c.ez = S.f[in_f + base];
-    c.mat = S.mat[base];
     if (x < S.nx && r < S.ny) {
         c.hx = S.f[in_f + 2 * S.ps + base];
         c.hy = S.f[in_f + 4 * S.ps + base];
     }
     const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (hi > lo && x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f)
+    if (hi > lo && x >= lo && x < hi)
         c.jp = S.jp[S.par * S.js + S.span[3 * r + 2] + x - lo];
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
@@ -132,7 +197,7 @@ __device__ __forceinline__ void store_cell(const SimView& S, const float4* __res
         S.f[out_f + 4 * S.ps + base] = c.hy;
     }
     const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (hi > lo && x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f)
+    if (hi > lo && x >= lo && x < hi)
         S.jp[(S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo] = c.jp;
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
@@ -372,7 +437,7 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
         __syncthreads();
         SimView S;
         S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
-        S.mat = (gcu8*)Sp->mat; S.span = (gcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (gcfloat*)Sp->srcw;
+        S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
         S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
         S.par = it.parity0 ^ (ordinal & 1);
         S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
@@ -403,11 +468,11 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
                 cur[t] = zero_cell();
                 hxp[t] = 0.f;
             }
-            Cell nxt;
-            load_cell(S, lc4, y0 - T, x, xin, wf, nxt);
-            for (int r = y0 - T; r < y1 + T; ++r) {
-                Cell in = nxt;
-                if (r + 1 < y1 + T) load_cell(S, lc4, r + 1, x, xin, wf, nxt);
+            auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
+            auto store = [&](int yo, const Cell& c) {
+                store_cell(S, lc4, yo, x, wf, c);
+            };
+            auto step = [&](int r, Cell in) {
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
                     const int y = r - 1 - t;
@@ -419,9 +484,9 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
                     cur[t] = in;
                     in = out;
                 }
-                const int yo = r - T;
-                if (yo >= y0 && yo < y1 && xown) store_cell(S, lc4, yo, x, wf, in);
-            }
+                return in;
+            };
+            row_stream<T, kGroupRef, Cell>(y0, y1, xown, load, store, step);
         }
     }
 }
@@ -467,7 +532,7 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         if (wv >= tile.nband) continue;  // after the last barrier of this tile
         SimView S;
         S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
-        S.mat = (gcu8*)Sp->mat; S.span = (gcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (gcfloat*)Sp->srcw;
+        S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
         S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
         S.par = it.parity0 ^ (ordinal & 1);
         S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
@@ -500,13 +565,11 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
             cur[t] = zero_cell();
             hxp[t] = 0.f;
         }
-        Cell n1, n2;
-        load_cell(S, lc4, y0 - T, x, xin, wf, n1);
-        load_cell(S, lc4, y0 - T + 1, x, xin, wf, n2);
-        for (int r = y0 - T; r < y1 + T; ++r) {
-            Cell in = n1;
-            n1 = n2;
-            if (r + 2 < y1 + T) load_cell(S, lc4, r + 2, x, xin, wf, n2);
+        auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
+        auto store = [&](int yo, const Cell& c) {
+                store_cell(S, lc4, yo, x, wf, c);
+        };
+        auto step = [&](int r, Cell in) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -547,9 +610,9 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
                 cur[t] = in;
                 in = c;
             }
-            const int yo = r - T;
-            if (yo >= y0 && yo < y1 && xown) store_cell(S, lc4, yo, x, wf, in);
-        }
+            return in;
+        };
+        row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step);
     }
 }
 
@@ -608,7 +671,7 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         gfloat* ph_in = (gfloat*)Sp->ybase + (2 + par) * ys;
         gfloat* ph_out = (gfloat*)Sp->ybase + (2 + (par ^ 1)) * ys;
         gcu8* mat = (gcu8*)Sp->mat;
-        gcint* span = (gcint*)Sp->span;
+        kcint* span = (kcint*)Sp->span;
         const float chx = Sp->chx, chy = Sp->chy, dyf = Sp->dyf;
         const int bx = tile.x0 + wv * V;
         const int x0 = bx - T;
@@ -616,7 +679,7 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         const bool own = lane >= T && x < min(bx + V, tile.xe);
         const int sx = Sp->sx, sy = Sp->sy;
         const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        kcfloat* srcw = (kcfloat*)Sp->srcw;
         // y-face NTFF rows (FDTD_common.cxx:4-46 legs yn, yp)
         const int D = Sp->D, P = Sp->P, K = Sp->chunk, L2 = Sp->L2, cx_hi = Sp->cx_hi;
         const bool xr = (x >= D + 1 && x <= cx_hi);
@@ -626,17 +689,16 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         const int k0 = ordinal * T;
         auto load = [&](int r, YCell& c) {
             c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
-            c.mat = 0;
+            c.mat = mat[min(max(r, 0), ny) * pitch + x];  // unconditional (see load_cell)
             if (r < 0 || r > ny) return;  // wave-uniform
             const int b = r * pitch + x;
             c.ez = fin[b];
-            c.mat = mat[b];
             if (r < ny) {
                 c.hx = fin[2 * ps + b];
                 c.hy = fin[4 * ps + b];
             }
             const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
             if (r >= 1 && r <= nc) c.pey = pe_in[(r - 1) * pitch + x];
             else if (r >= ny - nc && r <= ny - 1) c.pey = pe_in[(nc + r - (ny - nc)) * pitch + x];
             if (r < nc) c.phy = ph_in[r * pitch + x];
@@ -650,13 +712,21 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
             cur[t].mat = 0;
             hxp[t] = 0.f;
         }
-        YCell n1, n2;
-        load(y0 - T, n1);
-        load(y0 - T + 1, n2);
-        for (int r = y0 - T; r < y1 + T; ++r) {
-            YCell in = n1;
-            n1 = n2;
-            if (r + 2 < y1 + T) load(r + 2, n2);
+        auto store = [&](int yo, const YCell& c) {
+                const int b = yo * pitch + x;
+                fout[b] = c.ez;
+                if (yo < ny) {
+                    fout[2 * ps + b] = c.hx;
+                    fout[4 * ps + b] = c.hy;
+                }
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+                if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = c.pey;
+                else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = c.pey;
+                if (yo < nc) ph_out[yo * pitch + x] = c.phy;
+                else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = c.phy;
+        };
+        auto step = [&](int r, YCell in) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -701,22 +771,9 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
                 cur[t] = in;
                 in = c;
             }
-            const int yo = r - T;
-            if (yo >= y0 && yo < y1 && own) {
-                const int b = yo * pitch + x;
-                fout[b] = in.ez;
-                if (yo < ny) {
-                    fout[2 * ps + b] = in.hx;
-                    fout[4 * ps + b] = in.hy;
-                }
-                const int lo = span[3 * yo], hi = span[3 * yo + 1];
-                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
-                if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = in.pey;
-                else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = in.pey;
-                if (yo < nc) ph_out[yo * pitch + x] = in.phy;
-                else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = in.phy;
-            }
-        }
+            return in;
+        };
+        row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step);
     }
 }
 
@@ -773,7 +830,7 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         gfloat* ph_in = (gfloat*)Sp->xbase + (2 + par) * xs;
         gfloat* ph_out = (gfloat*)Sp->xbase + (2 + (par ^ 1)) * xs;
         gcu8* mat = (gcu8*)Sp->mat;
-        gcint* span = (gcint*)Sp->span;
+        kcint* span = (kcint*)Sp->span;
         const float chx = Sp->chx, chy = Sp->chy, dxf = Sp->dxf;
         const int bx = tile.x0 + wv * V;
         const int x0 = bx - T;
@@ -782,7 +839,7 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         const bool own = xin && lane >= T && x < min(bx + V, tile.xe);
         const int sx = Sp->sx, sy = Sp->sy;
         const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        kcfloat* srcw = (kcfloat*)Sp->srcw;
         // x-strip membership, psi storage column, coefficients, existence masks
         const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
         const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
@@ -804,17 +861,16 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         const int w2 = 2 * nc;
         auto load = [&](int r, XCell& c) {
             c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
-            c.mat = 0;
+            c.mat = mat[r * pitch + min(max(x, 0), nx)];  // unconditional (see load_cell)
             if (!xin) return;
             const int b = r * pitch + x;
             c.ez = fin[b];
-            c.mat = mat[b];
             if (hok) {
                 c.hx = fin[2 * ps + b];
                 c.hy = fin[4 * ps + b];
             }
             const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
             if (je >= 0) c.pex = pe_in[r * w2 + je];
             if (jh >= 0) c.phx = ph_in[r * w2 + jh];
         };
@@ -826,13 +882,19 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
             cur[t].mat = 0;
             hxp[t] = 0.f;
         }
-        XCell n1, n2;
-        load(y0 - T, n1);
-        load(y0 - T + 1, n2);
-        for (int r = y0 - T; r < y1 + T; ++r) {
-            XCell in = n1;
-            n1 = n2;
-            if (r + 2 < y1 + T) load(r + 2, n2);
+        auto store = [&](int yo, const XCell& c) {
+                const int b = yo * pitch + x;
+                fout[b] = c.ez;
+                if (hok) {
+                    fout[2 * ps + b] = c.hx;
+                    fout[4 * ps + b] = c.hy;
+                }
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+                if (je >= 0) pe_out[yo * w2 + je] = c.pex;
+                if (jh >= 0) ph_out[yo * w2 + jh] = c.phx;
+        };
+        auto step = [&](int r, XCell in) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -874,20 +936,9 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
                 cur[t] = in;
                 in = c;
             }
-            const int yo = r - T;
-            if (yo >= y0 && yo < y1 && own) {
-                const int b = yo * pitch + x;
-                fout[b] = in.ez;
-                if (hok) {
-                    fout[2 * ps + b] = in.hx;
-                    fout[4 * ps + b] = in.hy;
-                }
-                const int lo = span[3 * yo], hi = span[3 * yo + 1];
-                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
-                if (je >= 0) pe_out[yo * w2 + je] = in.pex;
-                if (jh >= 0) ph_out[yo * w2 + jh] = in.phx;
-            }
-        }
+            return in;
+        };
+        row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step);
     }
 }
 
@@ -929,14 +980,14 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
         gfloat* jin = (gfloat*)Sp->jbase + par * js;
         gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
         gcu8* mat = (gcu8*)Sp->mat;
-        gcint* span = (gcint*)Sp->span;
+        kcint* span = (kcint*)Sp->span;
         const float chx = Sp->chx, chy = Sp->chy;
         const int x0 = tile.x0 + wv * V - T;
         const int x = x0 + lane;
         const bool own = lane >= T && lane < kWave - T;
         const int sx = Sp->sx, sy = Sp->sy;
         const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        gcfloat* srcw = (gcfloat*)Sp->srcw;
+        kcfloat* srcw = (kcfloat*)Sp->srcw;
         const int y0 = tile.y0, y1 = tile.y1;
         const int step0 = it.step0 + ordinal * T;
         auto load = [&](int r, ICell& c) {
@@ -947,7 +998,15 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
             c.mat = mat[b];
             c.jp = 0.f;
             const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi && lc4[2 * c.mat + 1].w != 0.f) c.jp = jin[span[3 * r + 2] + x - lo];
+            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        };
+        auto store = [&](int yo, const ICell& c) {
+            const int b = yo * pitch + x;
+            fout[b] = c.ez;
+            fout[2 * ps + b] = c.hx;
+            fout[4 * ps + b] = c.hy;
+            const int lo = span[3 * yo], hi = span[3 * yo + 1];
+            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
         };
         ICell cur[T];
         float hxp[T];
@@ -957,13 +1016,7 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
             cur[t].mat = 0;
             hxp[t] = 0.f;
         }
-        ICell n1, n2;
-        load(y0 - T, n1);
-        load(y0 - T + 1, n2);
-        for (int r = y0 - T; r < y1 + T; ++r) {
-            ICell in = n1;
-            n1 = n2;
-            if (r + 2 < y1 + T) load(r + 2, n2);
+        auto step = [&](int r, ICell in) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -989,16 +1042,9 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
                 cur[t] = in;
                 in = c;
             }
-            const int yo = r - T;
-            if (yo >= y0 && yo < y1 && own) {
-                const int b = yo * pitch + x;
-                fout[b] = in.ez;
-                fout[2 * ps + b] = in.hx;
-                fout[4 * ps + b] = in.hy;
-                const int lo = span[3 * yo], hi = span[3 * yo + 1];
-                if (x >= lo && x < hi && lc4[2 * in.mat + 1].w != 0.f) jout[span[3 * yo + 2] + x - lo] = in.jp;
-            }
-        }
+            return in;
+        };
+        row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step);
     }
 }
 
