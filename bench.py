@@ -55,7 +55,7 @@ def cell_steps(cfg):
     return float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
 
 
-UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner")
+UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")
 
 
 def _kernel_roofline(name, k):
@@ -110,11 +110,11 @@ def kernel_ms(st, per=1.0):
     return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
 
 
-SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult")
+SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form")
 # Default schedules (picked by Context.autotune on MI355X; fixed so that the
 # profiled command and the timed one run the same launches).
-PLAIN_SCHEDULE = dict(tsteps=8, seg_rows=8, edge_waves=2, inner_waves=4, chunk=32, grid_mult=2)
-BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=8)
+PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=8, edge_waves=4, inner_waves=2, chunk=32, grid_mult=16, form=2)
+BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=16, form=1)
 
 
 def apply_schedule(ctx, cfg, n_sims, sched, tune):
@@ -269,7 +269,7 @@ def main():
     ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--schedule", type=str, default="",
-                    help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult")
+                    help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult,form")
     ap.add_argument("--batch-schedule", type=str, default="", help="batched-path schedule (same fields)")
     ap.add_argument("--autotune", action="store_true", help="pick both schedules with the autotuner instead")
     ap.add_argument("--plain-evals", type=int, default=2,
@@ -317,7 +317,7 @@ def main():
                           "s_per_batch": round(bdt, 3), "roofline": brl, "schedule": bsched,
                           "kernel_ms": kernel_ms(bst)}
         if args.plain_evals:
-            pl = run_plain_same_workload(ctx, args.plain_evals, psched if not tune else dict(zip(SCHED_KEYS, ctx.schedule())), bfits)
+            pl = run_plain_same_workload(ctx, args.plain_evals, sched, bfits)
             out["batched"]["plain_same_workload"] = pl
             out["batched"]["batched_vs_plain_evals_per_s"] = round(out["batched"]["evals_per_s"] / pl["evals_per_s"], 3)
     if args.serve:

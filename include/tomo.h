@@ -130,16 +130,21 @@ const char *tomo_last_error(void);
  * value).  tsteps: time steps fused per launch (1, 2, 4, 6, 8); seg_rows:
  * output rows per tile; edge_waves / inner_waves: waves per workgroup of the
  * edge and interior update kernels (1-4); chunk: running-DFT fold interval in
- * steps (a multiple of tsteps, <= 64); grid_mult: persistent workgroups per CU. */
+ * steps (a multiple of tsteps, <= 64); grid_mult: persistent workgroups per CU;
+ * form: how a physical-mode time block is launched, TOMO_FORM_STREAMS (one
+ * kernel per tile class on four streams, joined per block) or
+ * TOMO_FORM_FUSED (one launch of a kernel that runs every class). */
+#define TOMO_FORM_STREAMS 1
+#define TOMO_FORM_FUSED 2
 typedef struct tomo_schedule {
-    int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult;
+    int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form;
 } tomo_schedule;
 int tomo_set_schedule(tomo_ctx *ctx, const tomo_schedule *s);
 int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
 
 /* Autotuner: time candidate schedules on n_sims resident simulations of
- * `cfg` (coordinate descent over tsteps, seg_rows, edge/inner waves and
- * grid_mult; the chunk is kept), install the fastest and return it with its
+ * `cfg` (coordinate descent over form, tsteps, seg_rows, edge/inner waves
+ * and grid_mult; the chunk is kept), install the fastest and return it with its
  * measured update rate in cell-steps per second. */
 int tomo_autotune(tomo_ctx *ctx, const tomo_sim_config *cfg, int n_sims, tomo_schedule *best,
                   double *cell_steps_per_s);
@@ -160,7 +165,8 @@ enum {
     TOMO_KERNEL_FOLD = 5,    /* k_fold: running-DFT fold of a chunk of samples */
     TOMO_KERNEL_PROJECT = 6, /* k_project: far-field projection */
     TOMO_KERNEL_FITNESS = 7, /* k_fitness: FP64 fitness reduction */
-    TOMO_KERNEL_COUNT = 8
+    TOMO_KERNEL_FUSED = 8,   /* k_tfused: every tile class in one launch (TOMO_FORM_FUSED) */
+    TOMO_KERNEL_COUNT = 9
 };
 typedef struct tomo_kernel_stats {
     int64_t launches;

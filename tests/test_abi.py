@@ -74,3 +74,30 @@ def test_measurement_data_layout():
     c = t.measurement("cpu")
     assert g.size == 8000 and c.size == 4000
     assert np.all(g[:4000] >= 0) and np.all(np.abs(g[4000:]) <= np.pi)
+
+
+def test_header_constants_and_struct_layouts_match_the_python_mirror(tmp_path):
+    """Every #define and enum constant of include/tomo.h that the Python
+    mirror names has the same value, and the structs have the same sizes
+    (compiled with the host C compiler against the header itself)."""
+    text = open(os.path.join(ROOT, "include", "tomo.h")).read()
+    defs = dict(re.findall(r"^#define TOMO_([A-Z0-9_]+) (-?(?:0x)?[0-9A-Fa-f]+)\b", text, re.M))
+    defs.update(re.findall(r"TOMO_(KERNEL_[A-Z]+) = (\d+)", text))
+    checked = 0
+    for name, val in defs.items():
+        if hasattr(t, name):
+            assert getattr(t, name) == int(val, 0), name
+            checked += 1
+    assert checked >= 15
+    names = t.api.KERNEL_NAMES
+    assert len(names) == int(defs["KERNEL_COUNT"])
+    assert names[int(defs["KERNEL_FUSED"])] == "k_tfused" and names[int(defs["KERNEL_INNER"])] == "k_tinner"
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "tomo.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
+                   'sizeof(tomo_sim_config), sizeof(tomo_schedule), sizeof(tomo_stats), sizeof(tomo_kernel_stats));'
+                   'return 0;}\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True).stdout.split()]
+    assert got == [ctypes.sizeof(t.SimConfig), ctypes.sizeof(t.api.Schedule), ctypes.sizeof(t.api.Stats),
+                   ctypes.sizeof(t.api.KernelStats)]

@@ -84,13 +84,16 @@ def test_phys_fields_vs_oracle(ctx, oracle):
         assert rel_l2(a, b64) < FIELD_TOL_F64, (name, rel_l2(a, b64))
 
 
-@pytest.mark.parametrize("sched", [(1, 32, 4, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 3, 48), (8, 64, 2, 64)])
+@pytest.mark.parametrize("sched", [(1, 32, 4, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 3, 48), (8, 64, 2, 64),
+                                   (2, 8, 2, 32, 4, 2, t.FORM_FUSED), (8, 5, 4, 64, 1, 1, t.FORM_FUSED),
+                                   (4, 16, 1, 32, 3, 8, t.FORM_FUSED)])
 def test_schedule_does_not_change_results(oracle, sched):
-    """Fused step count, tile height, waves per workgroup and DFT chunk only
-    reorder work: every schedule gives the same fields and accumulators."""
+    """Fused step count, tile height, waves per workgroup, DFT chunk and the
+    launch form only reorder work: every schedule gives the same fields and
+    accumulators."""
     cfg = phys_small(steps=200)
     base = t.Context()
-    base.set_schedule(4, 32, 4, 32)
+    base.set_schedule(4, 32, 4, 32, form=t.FORM_STREAMS)
     a = base.sim(cfg, PHYS_PARAMS, exc=0)
     a.step(200)
     other = t.Context()
@@ -236,25 +239,28 @@ def test_autotune_installs_a_schedule_that_keeps_results(oracle):
     c.close()
 
 
-def test_kernel_stats_account_for_every_cell_step():
+@pytest.mark.parametrize("form", [1, 2])
+def test_kernel_stats_account_for_every_cell_step(form):
     """Per-kernel timing: the update kernels' cell-steps add up to the
-    simulations' cell-steps, every tile class of a split grid runs, and the
-    algorithmic bytes per cell-step stay within the 25..49 bytes the formula
-    allows."""
+    simulations' cell-steps, every tile class of a split grid runs (in its
+    own kernel, or inside the fused one), and the algorithmic bytes per
+    cell-step stay within the 25..49 bytes the formula allows."""
     cfg = phys_small(steps=96, n=256, n_exc=2)
     c = t.Context()
-    c.set_schedule(4, 16, 2, 32, 2, 2)
+    c.set_schedule(4, 16, 2, 32, 2, 2, form)
     c.target_farfield(cfg)  # the target's own simulations stay out of the counters
     c.profile(True)
     c.eval(cfg, PHYS_PARAMS)
     st = c.stats()
-    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner")]
+    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")]
     want = float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
     assert sum(k["cell_steps"] for k in upd) == want
     assert st["update"]["cell_steps"] == want
-    for k in ("k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness"):
+    ran = ("k_tedge", "k_trow", "k_tcol", "k_tinner") if form == 1 else ("k_tfused",)
+    for k in ran + ("k_fold", "k_project", "k_fitness"):
         assert st[k]["launches"] > 0 and st[k]["ms"] > 0, k
-    assert st["k_tblock"]["launches"] == 0
+    for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")) - set(ran):
+        assert st[k]["launches"] == 0, k
     b = sum(k["alg_bytes"] for k in upd) / want
     assert 25.0 < b < 49.0
     c.close()

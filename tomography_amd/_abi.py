@@ -32,6 +32,9 @@ PRESET_CFG3 = 4
 PRESET_CFG4 = 5
 PRESET_CFG5 = 6
 
+FORM_STREAMS = 1
+FORM_FUSED = 2
+
 
 class SimConfig(_c.Structure):
     """tomo_sim_config (include/tomo.h)."""

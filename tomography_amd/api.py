@@ -32,13 +32,15 @@ class Schedule(_c.Structure):
     """tomo_schedule (include/tomo.h)."""
 
     _fields_ = [("tsteps", _c.c_int32), ("seg_rows", _c.c_int32), ("edge_waves", _c.c_int32),
-                ("inner_waves", _c.c_int32), ("chunk", _c.c_int32), ("grid_mult", _c.c_int32)]
+                ("inner_waves", _c.c_int32), ("chunk", _c.c_int32), ("grid_mult", _c.c_int32),
+                ("form", _c.c_int32)]
 
     def as_tuple(self):
         return tuple(getattr(self, k) for k, _ in self._fields_)
 
 
-KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness")
+KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness",
+                "k_tfused")
 
 
 class KernelStats(_c.Structure):
@@ -153,8 +155,8 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def set_schedule(self, tsteps=0, seg_rows=0, edge_waves=0, chunk=0, inner_waves=0, grid_mult=0):
-        s = Schedule(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult)
+    def set_schedule(self, tsteps=0, seg_rows=0, edge_waves=0, chunk=0, inner_waves=0, grid_mult=0, form=0):
+        s = Schedule(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form)
         _check(lib().tomo_set_schedule(self.h, _c.byref(s)))
 
     def autotune(self, cfg, n_sims):
@@ -173,7 +175,7 @@ class Context:
         return st.as_dict()
 
     def schedule(self):
-        """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult)"""
+        """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form)"""
         s = Schedule()
         _check(lib().tomo_get_schedule(self.h, _c.byref(s)))
         return s.as_tuple()

@@ -23,6 +23,9 @@ hipError_t launch_tcol(int T, const TileItem* tiles, int ntiles, const LaunchIte
                        int ordinal, int waves, int grid, hipStream_t st);
 hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
                          int ordinal, int waves, int grid, hipStream_t st);
+// All tile classes of physical-mode simulations in one launch (TileItem::cls).
+hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                         int ordinal, int waves, int grid, hipStream_t st);
 
 // Fold one chunk of surface samples into the running DFT (Kahan FP32 or FP64).
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,

@@ -499,15 +499,119 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
 // compute exactly what k_tinner computes (multiplications by 1 and additions
 // of 0 are exact).
 template <int T>
+__device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcp) {
+    {
+        float* lcf = (float*)lc4;
+        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+    }
+    for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+        const bool id = (i >= Sp->nc);
+        CP(2, i) = id ? 1.f : Sp->ike[i];
+        CP(3, i) = id ? 1.f : Sp->ikm[i];
+        CP(4, i) = id ? 0.f : Sp->ae[i];
+        CP(5, i) = id ? 0.f : Sp->be[i];
+        CP(6, i) = id ? 0.f : Sp->am[i];
+        CP(7, i) = id ? 0.f : Sp->bm[i];
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem& it, int nst,
+                                         const DevSim* __restrict__ Sp, int ordinal,
+                                         const float4* __restrict__ lc4, const float* __restrict__ lcp) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    constexpr int V = kWave - 2 * T;
+    constexpr int ID = TOMO_MAX_NC - 1;
+    SimView S;
+    S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
+    S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
+    S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
+    S.par = it.parity0 ^ (ordinal & 1);
+    S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
+    S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
+    S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
+    S.chx = Sp->chx; S.chy = Sp->chy; S.dxf = Sp->dxf; S.dyf = Sp->dyf;
+    const int nx = S.nx, ny = S.ny, nc = S.nc;
+    const int bx = tile.x0 + wv * V;
+    const int x0 = bx - T;
+    const int x = x0 + lane;
+    const bool xin = (x >= 0 && x <= nx);
+    const bool xown = xin && lane >= T && x < min(bx + V, tile.xe);
+    WaveFlags wf;
+    wf.xs = (x0 <= nc) || (x0 + kWave - 1 >= nx - nc);
+    wf.ccol = (x0 <= S.D + 1 && x0 + kWave - 1 >= S.D) || (x0 <= nx - S.D && x0 + kWave - 1 >= nx - S.D - 1);
+    wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
+    // lane constants: x-strip coefficients and existence masks
+    const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+    const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+    const float kxe = CP(2, ixe), axe = CP(4, ixe), bxe = CP(5, ixe);
+    const float kxh = CP(3, ixh), axh = CP(6, ixh), bxh = CP(7, ixh);
+    const bool hx_ok = (x >= 0 && x < nx), ex_ok = (x >= 1 && x <= nx - 1);
+    const int y0 = tile.y0, y1 = tile.y1;
+    const int step0 = it.step0 + ordinal * T;
+    const int k0 = ordinal * T;
+    Cell cur[T];
+    float hxp[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        cur[t] = zero_cell();
+        hxp[t] = 0.f;
+    }
+    auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
+    auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c); };
+    auto step = [&](int r, Cell in) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int y = r - 1 - t;
+            Cell c = cur[t];
+            if (t < nst) {
+                // wave-uniform row coefficients
+                const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
+                // H half-step with the CPML recursions
+                const float dEx = lane_next(c.ez) - c.ez;
+                const float dEy = in.ez - c.ez;
+                c.phx = bxh * c.phx + axh * dEx;
+                c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+                const float ty = dEx * kxh + S.dxf * c.phx;
+                const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
+                const bool hok = hx_ok && hy_ok;
+                const float hy = hok ? c.hy + S.chy * ty : 0.f;
+                const float hx = hok ? c.hx - S.chx * tx : 0.f;
+                // E half-step with the CPML recursions and the Debye current
+                const float dHy = hy - lane_prev(hy);
+                const float dHx = hx - hxp[t];
+                hxp[t] = hx;
+                c.pex = bxe * c.pex + axe * dHy;
+                c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+                const float cy = dHy * kxe + S.dxf * c.pex;
+                const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+                const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
+                if (wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
+                const bool eok = ex_ok && ey_ok;
+                c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                c.ez = eok ? en : 0.f;
+                c.hx = hx;
+                c.hy = hy;
+                if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
+            }
+            cur[t] = in;
+            in = c;
+        }
+        return in;
+    };
+    row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step);
+}
+
+template <int T>
 __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tiles, int ntiles,
                                                const LaunchItem* __restrict__ items,
                                                const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     __shared__ float lcp[8 * TOMO_MAX_NC];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    constexpr int V = kWave - 2 * T;
-    constexpr int ID = TOMO_MAX_NC - 1;  // identity entry of the CPML tables
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -515,106 +619,13 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        {
-            float* lcf = (float*)lc4;
-            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
-        }
-        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
-            const bool id = (i >= Sp->nc);
-            CP(2, i) = id ? 1.f : Sp->ike[i];
-            CP(3, i) = id ? 1.f : Sp->ikm[i];
-            CP(4, i) = id ? 0.f : Sp->ae[i];
-            CP(5, i) = id ? 0.f : Sp->be[i];
-            CP(6, i) = id ? 0.f : Sp->am[i];
-            CP(7, i) = id ? 0.f : Sp->bm[i];
-        }
+        fill_edge<T>(Sp, lc4, lcp);
         __syncthreads();
-        if (wv >= tile.nband) continue;  // after the last barrier of this tile
-        SimView S;
-        S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
-        S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
-        S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
-        S.par = it.parity0 ^ (ordinal & 1);
-        S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
-        S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
-        S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
-        S.chx = Sp->chx; S.chy = Sp->chy; S.dxf = Sp->dxf; S.dyf = Sp->dyf;
-        const int nx = S.nx, ny = S.ny, nc = S.nc;
-        const int bx = tile.x0 + wv * V;
-        const int x0 = bx - T;
-        const int x = x0 + lane;
-        const bool xin = (x >= 0 && x <= nx);
-        const bool xown = xin && lane >= T && x < min(bx + V, tile.xe);
-        WaveFlags wf;
-        wf.xs = (x0 <= nc) || (x0 + kWave - 1 >= nx - nc);
-        wf.ccol = (x0 <= S.D + 1 && x0 + kWave - 1 >= S.D) || (x0 <= nx - S.D && x0 + kWave - 1 >= nx - S.D - 1);
-        wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
-        // lane constants: x-strip coefficients and existence masks
-        const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
-        const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
-        const float kxe = CP(2, ixe), axe = CP(4, ixe), bxe = CP(5, ixe);
-        const float kxh = CP(3, ixh), axh = CP(6, ixh), bxh = CP(7, ixh);
-        const bool hx_ok = (x >= 0 && x < nx), ex_ok = (x >= 1 && x <= nx - 1);
-        const int y0 = tile.y0, y1 = tile.y1;
-        const int step0 = it.step0 + ordinal * T;
-        const int k0 = ordinal * T;
-        Cell cur[T];
-        float hxp[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            cur[t] = zero_cell();
-            hxp[t] = 0.f;
-        }
-        auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
-        auto store = [&](int yo, const Cell& c) {
-                store_cell(S, lc4, yo, x, wf, c);
-        };
-        auto step = [&](int r, Cell in) {
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int y = r - 1 - t;
-                Cell c = cur[t];
-                if (t < nst) {
-                    // wave-uniform row coefficients
-                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
-                    const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
-                    // H half-step with the CPML recursions
-                    const float dEx = lane_next(c.ez) - c.ez;
-                    const float dEy = in.ez - c.ez;
-                    c.phx = bxh * c.phx + axh * dEx;
-                    c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
-                    const float ty = dEx * kxh + S.dxf * c.phx;
-                    const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
-                    const bool hok = hx_ok && hy_ok;
-                    const float hy = hok ? c.hy + S.chy * ty : 0.f;
-                    const float hx = hok ? c.hx - S.chx * tx : 0.f;
-                    // E half-step with the CPML recursions and the Debye current
-                    const float dHy = hy - lane_prev(hy);
-                    const float dHx = hx - hxp[t];
-                    hxp[t] = hx;
-                    c.pex = bxe * c.pex + axe * dHy;
-                    c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
-                    const float cy = dHy * kxe + S.dxf * c.pex;
-                    const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
-                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                    float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
-                    if (wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
-                    const bool eok = ex_ok && ey_ok;
-                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
-                    c.ez = eok ? en : 0.f;
-                    c.hx = hx;
-                    c.hy = hy;
-                    if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
-                }
-                cur[t] = in;
-                in = c;
-            }
-            return in;
-        };
-        row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step);
+        if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        body_edge<T>(tile, it, nst, Sp, ordinal, lc4, lcp);
     }
 }
+
 
 #undef CP
 
@@ -629,15 +640,153 @@ struct YCell {
 };
 
 template <int T>
-__global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles, int ntiles,
-                                              const LaunchItem* __restrict__ items,
-                                              const DevSim* __restrict__ sims, int ordinal) {
-    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
-    __shared__ float lcy[6 * TOMO_MAX_NC];  // ike ikm ae be am bm with an identity entry
+__device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcy) {
+    {
+        float* lcf = (float*)lc4;
+        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+    }
+    for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+        const bool id = (i >= Sp->nc);
+        lcy[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
+        lcy[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
+        lcy[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
+        lcy[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
+        lcy[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
+        lcy[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem& it, int nst,
+                                         const DevSim* __restrict__ Sp, int ordinal,
+                                         const float4* __restrict__ lc4, const float* __restrict__ lcy) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     constexpr int V = kWave - 2 * T;
     constexpr int ID = TOMO_MAX_NC - 1;
+    const int par = it.parity0 ^ (ordinal & 1);
+    const int ps = Sp->ps, js = Sp->js, ys = Sp->ys, pitch = Sp->pitch, nx = Sp->nx, ny = Sp->ny, nc = Sp->nc;
+    gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+    gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+    gfloat* jin = (gfloat*)Sp->jbase + par * js;
+    gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+    gfloat* pe_in = (gfloat*)Sp->ybase + par * ys;
+    gfloat* pe_out = (gfloat*)Sp->ybase + (par ^ 1) * ys;
+    gfloat* ph_in = (gfloat*)Sp->ybase + (2 + par) * ys;
+    gfloat* ph_out = (gfloat*)Sp->ybase + (2 + (par ^ 1)) * ys;
+    gcu8* mat = (gcu8*)Sp->mat;
+    kcint* span = (kcint*)Sp->span;
+    const float chx = Sp->chx, chy = Sp->chy, dyf = Sp->dyf;
+    const int bx = tile.x0 + wv * V;
+    const int x0 = bx - T;
+    const int x = x0 + lane;
+    const bool own = lane >= T && x < min(bx + V, tile.xe);
+    const int sx = Sp->sx, sy = Sp->sy;
+    const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+    kcfloat* srcw = (kcfloat*)Sp->srcw;
+    // y-face NTFF rows (FDTD_common.cxx:4-46 legs yn, yp)
+    const int D = Sp->D, P = Sp->P, K = Sp->chunk, L2 = Sp->L2, cx_hi = Sp->cx_hi;
+    const bool xr = (x >= D + 1 && x <= cx_hi);
+    gfloat* samp = (gfloat*)Sp->samp;
+    const int y0 = tile.y0, y1 = tile.y1;
+    const int step0 = it.step0 + ordinal * T;
+    const int k0 = ordinal * T;
+    auto load = [&](int r, YCell& c) {
+        c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
+        c.mat = mat[min(max(r, 0), ny) * pitch + x];  // unconditional (see load_cell)
+        if (r < 0 || r > ny) return;  // wave-uniform
+        const int b = r * pitch + x;
+        c.ez = fin[b];
+        if (r < ny) {
+            c.hx = fin[2 * ps + b];
+            c.hy = fin[4 * ps + b];
+        }
+        const int lo = span[3 * r], hi = span[3 * r + 1];
+        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        if (r >= 1 && r <= nc) c.pey = pe_in[(r - 1) * pitch + x];
+        else if (r >= ny - nc && r <= ny - 1) c.pey = pe_in[(nc + r - (ny - nc)) * pitch + x];
+        if (r < nc) c.phy = ph_in[r * pitch + x];
+        else if (r >= ny - nc && r < ny) c.phy = ph_in[(nc + r - (ny - nc)) * pitch + x];
+    };
+    YCell cur[T];
+    float hxp[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pey = cur[t].phy = 0.f;
+        cur[t].mat = 0;
+        hxp[t] = 0.f;
+    }
+    auto store = [&](int yo, const YCell& c) {
+            const int b = yo * pitch + x;
+            fout[b] = c.ez;
+            if (yo < ny) {
+                fout[2 * ps + b] = c.hx;
+                fout[4 * ps + b] = c.hy;
+            }
+            const int lo = span[3 * yo], hi = span[3 * yo + 1];
+            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = c.pey;
+            else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = c.pey;
+            if (yo < nc) ph_out[yo * pitch + x] = c.phy;
+            else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = c.phy;
+    };
+    auto step = [&](int r, YCell in) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int y = r - 1 - t;
+            YCell c = cur[t];
+            if (t < nst) {
+                const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
+                const float dEx = lane_next(c.ez) - c.ez;
+                const float dEy = in.ez - c.ez;
+                c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
+                const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
+                const float hy = hok ? c.hy + chy * dEx : 0.f;
+                const float hx = hok ? c.hx - chx * tx : 0.f;
+                const float dHy = hy - lane_prev(hy);
+                const float dHx = hx - hxp[t];
+                hxp[t] = hx;
+                c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
+                const float cx = dHx * lcy[iye] + dyf * c.pey;
+                const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
+                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                c.ez = eok ? en : 0.f;
+                c.hx = hx;
+                c.hy = hy;
+                if ((y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 && y < y1 && own && xr) {
+                    const int k = k0 + t;
+                    if (y == D) {
+                        samp[k * P + x - D - 1] = c.hx;
+                        samp[(K + k) * P + x - D - 1] = c.ez;
+                    } else if (y == D + 1) {
+                        samp[(2 * K + k) * P + x - D - 1] = c.ez;
+                    } else if (y == ny - D - 1) {
+                        samp[k * P + L2 + (cx_hi - x)] = c.hx;
+                        samp[(K + k) * P + L2 + (cx_hi - x)] = c.ez;
+                    } else {
+                        samp[(2 * K + k) * P + L2 + (cx_hi - x)] = c.ez;
+                    }
+                }
+            }
+            cur[t] = in;
+            in = c;
+        }
+        return in;
+    };
+    row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step);
+
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles, int ntiles,
+                                               const LaunchItem* __restrict__ items,
+                                               const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcy[6 * TOMO_MAX_NC];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -645,137 +794,13 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        {
-            float* lcf = (float*)lc4;
-            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
-        }
-        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
-            const bool id = (i >= Sp->nc);
-            lcy[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
-            lcy[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
-            lcy[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
-            lcy[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
-            lcy[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
-            lcy[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
-        }
+        fill_row<T>(Sp, lc4, lcy);
         __syncthreads();
-        if (wv >= tile.nband) continue;  // after the last barrier of this tile
-        const int par = it.parity0 ^ (ordinal & 1);
-        const int ps = Sp->ps, js = Sp->js, ys = Sp->ys, pitch = Sp->pitch, nx = Sp->nx, ny = Sp->ny, nc = Sp->nc;
-        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
-        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
-        gfloat* jin = (gfloat*)Sp->jbase + par * js;
-        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
-        gfloat* pe_in = (gfloat*)Sp->ybase + par * ys;
-        gfloat* pe_out = (gfloat*)Sp->ybase + (par ^ 1) * ys;
-        gfloat* ph_in = (gfloat*)Sp->ybase + (2 + par) * ys;
-        gfloat* ph_out = (gfloat*)Sp->ybase + (2 + (par ^ 1)) * ys;
-        gcu8* mat = (gcu8*)Sp->mat;
-        kcint* span = (kcint*)Sp->span;
-        const float chx = Sp->chx, chy = Sp->chy, dyf = Sp->dyf;
-        const int bx = tile.x0 + wv * V;
-        const int x0 = bx - T;
-        const int x = x0 + lane;
-        const bool own = lane >= T && x < min(bx + V, tile.xe);
-        const int sx = Sp->sx, sy = Sp->sy;
-        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        kcfloat* srcw = (kcfloat*)Sp->srcw;
-        // y-face NTFF rows (FDTD_common.cxx:4-46 legs yn, yp)
-        const int D = Sp->D, P = Sp->P, K = Sp->chunk, L2 = Sp->L2, cx_hi = Sp->cx_hi;
-        const bool xr = (x >= D + 1 && x <= cx_hi);
-        gfloat* samp = (gfloat*)Sp->samp;
-        const int y0 = tile.y0, y1 = tile.y1;
-        const int step0 = it.step0 + ordinal * T;
-        const int k0 = ordinal * T;
-        auto load = [&](int r, YCell& c) {
-            c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
-            c.mat = mat[min(max(r, 0), ny) * pitch + x];  // unconditional (see load_cell)
-            if (r < 0 || r > ny) return;  // wave-uniform
-            const int b = r * pitch + x;
-            c.ez = fin[b];
-            if (r < ny) {
-                c.hx = fin[2 * ps + b];
-                c.hy = fin[4 * ps + b];
-            }
-            const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
-            if (r >= 1 && r <= nc) c.pey = pe_in[(r - 1) * pitch + x];
-            else if (r >= ny - nc && r <= ny - 1) c.pey = pe_in[(nc + r - (ny - nc)) * pitch + x];
-            if (r < nc) c.phy = ph_in[r * pitch + x];
-            else if (r >= ny - nc && r < ny) c.phy = ph_in[(nc + r - (ny - nc)) * pitch + x];
-        };
-        YCell cur[T];
-        float hxp[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pey = cur[t].phy = 0.f;
-            cur[t].mat = 0;
-            hxp[t] = 0.f;
-        }
-        auto store = [&](int yo, const YCell& c) {
-                const int b = yo * pitch + x;
-                fout[b] = c.ez;
-                if (yo < ny) {
-                    fout[2 * ps + b] = c.hx;
-                    fout[4 * ps + b] = c.hy;
-                }
-                const int lo = span[3 * yo], hi = span[3 * yo + 1];
-                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
-                if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = c.pey;
-                else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = c.pey;
-                if (yo < nc) ph_out[yo * pitch + x] = c.phy;
-                else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = c.phy;
-        };
-        auto step = [&](int r, YCell in) {
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int y = r - 1 - t;
-                YCell c = cur[t];
-                if (t < nst) {
-                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
-                    const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
-                    const float dEx = lane_next(c.ez) - c.ez;
-                    const float dEy = in.ez - c.ez;
-                    c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
-                    const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
-                    const float hy = hok ? c.hy + chy * dEx : 0.f;
-                    const float hx = hok ? c.hx - chx * tx : 0.f;
-                    const float dHy = hy - lane_prev(hy);
-                    const float dHx = hx - hxp[t];
-                    hxp[t] = hx;
-                    c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
-                    const float cx = dHx * lcy[iye] + dyf * c.pey;
-                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                    float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
-                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
-                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
-                    c.ez = eok ? en : 0.f;
-                    c.hx = hx;
-                    c.hy = hy;
-                    if ((y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 && y < y1 && own && xr) {
-                        const int k = k0 + t;
-                        if (y == D) {
-                            samp[k * P + x - D - 1] = c.hx;
-                            samp[(K + k) * P + x - D - 1] = c.ez;
-                        } else if (y == D + 1) {
-                            samp[(2 * K + k) * P + x - D - 1] = c.ez;
-                        } else if (y == ny - D - 1) {
-                            samp[k * P + L2 + (cx_hi - x)] = c.hx;
-                            samp[(K + k) * P + L2 + (cx_hi - x)] = c.ez;
-                        } else {
-                            samp[(2 * K + k) * P + L2 + (cx_hi - x)] = c.ez;
-                        }
-                    }
-                }
-                cur[t] = in;
-                in = c;
-            }
-            return in;
-        };
-        row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step);
+        if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        body_row<T>(tile, it, nst, Sp, ordinal, lc4, lcy);
     }
 }
+
 
 // Column-edge tiles of physical-mode simulations: bands that touch an x-side
 // CPML strip, x-face NTFF column or x edge, over rows whose ghost zone is
@@ -788,15 +813,159 @@ struct XCell {
 };
 
 template <int T>
-__global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles, int ntiles,
-                                              const LaunchItem* __restrict__ items,
-                                              const DevSim* __restrict__ sims, int ordinal) {
-    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
-    __shared__ float lcx[6 * TOMO_MAX_NC];  // ike ikm ae be am bm with an identity entry
+__device__ __forceinline__ void fill_col(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcx) {
+    {
+        float* lcf = (float*)lc4;
+        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+    }
+    for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+        const bool id = (i >= Sp->nc);
+        lcx[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
+        lcx[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
+        lcx[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
+        lcx[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
+        lcx[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
+        lcx[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem& it, int nst,
+                                         const DevSim* __restrict__ Sp, int ordinal,
+                                         const float4* __restrict__ lc4, const float* __restrict__ lcx) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     constexpr int V = kWave - 2 * T;
     constexpr int ID = TOMO_MAX_NC - 1;
+    const int par = it.parity0 ^ (ordinal & 1);
+    const int ps = Sp->ps, js = Sp->js, xs = Sp->xs, pitch = Sp->pitch, nx = Sp->nx, nc = Sp->nc;
+    gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+    gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+    gfloat* jin = (gfloat*)Sp->jbase + par * js;
+    gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+    gfloat* pe_in = (gfloat*)Sp->xbase + par * xs;
+    gfloat* pe_out = (gfloat*)Sp->xbase + (par ^ 1) * xs;
+    gfloat* ph_in = (gfloat*)Sp->xbase + (2 + par) * xs;
+    gfloat* ph_out = (gfloat*)Sp->xbase + (2 + (par ^ 1)) * xs;
+    gcu8* mat = (gcu8*)Sp->mat;
+    kcint* span = (kcint*)Sp->span;
+    const float chx = Sp->chx, chy = Sp->chy, dxf = Sp->dxf;
+    const int bx = tile.x0 + wv * V;
+    const int x0 = bx - T;
+    const int x = x0 + lane;
+    const bool xin = (x >= 0 && x <= nx);
+    const bool own = xin && lane >= T && x < min(bx + V, tile.xe);
+    const int sx = Sp->sx, sy = Sp->sy;
+    const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+    kcfloat* srcw = (kcfloat*)Sp->srcw;
+    // x-strip membership, psi storage column, coefficients, existence masks
+    const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+    const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+    const int je = (x >= 1 && x <= nc) ? x - 1 : ((x >= nx - nc && x <= nx - 1) ? nc + x - (nx - nc) : -1);
+    const int jh = (x >= 0 && x < nc) ? x : ((x >= nx - nc && x < nx) ? nc + x - (nx - nc) : -1);
+    const float kxe = lcx[ixe], kxh = lcx[TOMO_MAX_NC + ixh];
+    const float axe = lcx[2 * TOMO_MAX_NC + ixe], bxe = lcx[3 * TOMO_MAX_NC + ixe];
+    const float axh = lcx[4 * TOMO_MAX_NC + ixh], bxh = lcx[5 * TOMO_MAX_NC + ixh];
+    const bool hok = (x >= 0 && x < nx), eok = (x >= 1 && x <= nx - 1);
+    // x-face NTFF contour columns (FDTD_common.cxx:4-46 legs xp, xn)
+    const int D = Sp->D, ny = Sp->ny, P = Sp->P, K = Sp->chunk;
+    const bool ccol = (x == nx - D - 1 || x == nx - D || x == D || x == D + 1);
+    const bool wcc = (x0 <= D + 1 && x0 + kWave - 1 >= D) || (x0 <= nx - D && x0 + kWave - 1 >= nx - D - 1);
+    gfloat* samp = (gfloat*)Sp->samp;
+    const int L1 = Sp->L1, L3 = Sp->L3, cy_lo = Sp->cy_lo;
+    const int y0 = tile.y0, y1 = tile.y1;
+    const int step0 = it.step0 + ordinal * T;
+    const int k0 = ordinal * T;
+    const int w2 = 2 * nc;
+    auto load = [&](int r, XCell& c) {
+        c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
+        c.mat = mat[r * pitch + min(max(x, 0), nx)];  // unconditional (see load_cell)
+        if (!xin) return;
+        const int b = r * pitch + x;
+        c.ez = fin[b];
+        if (hok) {
+            c.hx = fin[2 * ps + b];
+            c.hy = fin[4 * ps + b];
+        }
+        const int lo = span[3 * r], hi = span[3 * r + 1];
+        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        if (je >= 0) c.pex = pe_in[r * w2 + je];
+        if (jh >= 0) c.phx = ph_in[r * w2 + jh];
+    };
+    XCell cur[T];
+    float hxp[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pex = cur[t].phx = 0.f;
+        cur[t].mat = 0;
+        hxp[t] = 0.f;
+    }
+    auto store = [&](int yo, const XCell& c) {
+            const int b = yo * pitch + x;
+            fout[b] = c.ez;
+            if (hok) {
+                fout[2 * ps + b] = c.hx;
+                fout[4 * ps + b] = c.hy;
+            }
+            const int lo = span[3 * yo], hi = span[3 * yo + 1];
+            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            if (je >= 0) pe_out[yo * w2 + je] = c.pex;
+            if (jh >= 0) ph_out[yo * w2 + jh] = c.phx;
+    };
+    auto step = [&](int r, XCell in) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int y = r - 1 - t;
+            XCell c = cur[t];
+            if (t < nst) {
+                const float dEx = lane_next(c.ez) - c.ez;
+                const float dEy = in.ez - c.ez;
+                c.phx = bxh * c.phx + axh * dEx;
+                const float ty = dEx * kxh + dxf * c.phx;
+                const float hy = hok ? c.hy + chy * ty : 0.f;
+                const float hx = hok ? c.hx - chx * dEy : 0.f;
+                const float dHy = hy - lane_prev(hy);
+                const float dHx = hx - hxp[t];
+                hxp[t] = hx;
+                c.pex = bxe * c.pex + axe * dHy;
+                const float cy = dHy * kxe + dxf * c.pex;
+                const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                float en = qa.x * c.ez + qa.y * cy - qa.y * dHx - qb.x * c.jp;
+                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                c.ez = eok ? en : 0.f;
+                c.hx = hx;
+                c.hy = hy;
+                if (wcc && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
+                    const int k = k0 + t;
+                    if (x == nx - D - 1) {
+                        samp[k * P + L1 + y - cy_lo] = c.hy;
+                        samp[(K + k) * P + L1 + y - cy_lo] = c.ez;
+                    } else if (x == nx - D) {
+                        samp[(2 * K + k) * P + L1 + y - cy_lo] = c.ez;
+                    } else if (x == D) {
+                        samp[k * P + L3 + (ny - D - 1 - y)] = c.hy;
+                        samp[(K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
+                    } else {
+                        samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
+                    }
+                }
+            }
+            cur[t] = in;
+            in = c;
+        }
+        return in;
+    };
+    row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step);
+
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles, int ntiles,
+                                               const LaunchItem* __restrict__ items,
+                                               const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcx[6 * TOMO_MAX_NC];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -804,143 +973,13 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        {
-            float* lcf = (float*)lc4;
-            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
-        }
-        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
-            const bool id = (i >= Sp->nc);
-            lcx[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
-            lcx[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
-            lcx[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
-            lcx[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
-            lcx[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
-            lcx[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
-        }
+        fill_col<T>(Sp, lc4, lcx);
         __syncthreads();
-        if (wv >= tile.nband) continue;  // after the last barrier of this tile
-        const int par = it.parity0 ^ (ordinal & 1);
-        const int ps = Sp->ps, js = Sp->js, xs = Sp->xs, pitch = Sp->pitch, nx = Sp->nx, nc = Sp->nc;
-        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
-        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
-        gfloat* jin = (gfloat*)Sp->jbase + par * js;
-        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
-        gfloat* pe_in = (gfloat*)Sp->xbase + par * xs;
-        gfloat* pe_out = (gfloat*)Sp->xbase + (par ^ 1) * xs;
-        gfloat* ph_in = (gfloat*)Sp->xbase + (2 + par) * xs;
-        gfloat* ph_out = (gfloat*)Sp->xbase + (2 + (par ^ 1)) * xs;
-        gcu8* mat = (gcu8*)Sp->mat;
-        kcint* span = (kcint*)Sp->span;
-        const float chx = Sp->chx, chy = Sp->chy, dxf = Sp->dxf;
-        const int bx = tile.x0 + wv * V;
-        const int x0 = bx - T;
-        const int x = x0 + lane;
-        const bool xin = (x >= 0 && x <= nx);
-        const bool own = xin && lane >= T && x < min(bx + V, tile.xe);
-        const int sx = Sp->sx, sy = Sp->sy;
-        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        kcfloat* srcw = (kcfloat*)Sp->srcw;
-        // x-strip membership, psi storage column, coefficients, existence masks
-        const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
-        const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
-        const int je = (x >= 1 && x <= nc) ? x - 1 : ((x >= nx - nc && x <= nx - 1) ? nc + x - (nx - nc) : -1);
-        const int jh = (x >= 0 && x < nc) ? x : ((x >= nx - nc && x < nx) ? nc + x - (nx - nc) : -1);
-        const float kxe = lcx[ixe], kxh = lcx[TOMO_MAX_NC + ixh];
-        const float axe = lcx[2 * TOMO_MAX_NC + ixe], bxe = lcx[3 * TOMO_MAX_NC + ixe];
-        const float axh = lcx[4 * TOMO_MAX_NC + ixh], bxh = lcx[5 * TOMO_MAX_NC + ixh];
-        const bool hok = (x >= 0 && x < nx), eok = (x >= 1 && x <= nx - 1);
-        // x-face NTFF contour columns (FDTD_common.cxx:4-46 legs xp, xn)
-        const int D = Sp->D, ny = Sp->ny, P = Sp->P, K = Sp->chunk;
-        const bool ccol = (x == nx - D - 1 || x == nx - D || x == D || x == D + 1);
-        const bool wcc = (x0 <= D + 1 && x0 + kWave - 1 >= D) || (x0 <= nx - D && x0 + kWave - 1 >= nx - D - 1);
-        gfloat* samp = (gfloat*)Sp->samp;
-        const int L1 = Sp->L1, L3 = Sp->L3, cy_lo = Sp->cy_lo;
-        const int y0 = tile.y0, y1 = tile.y1;
-        const int step0 = it.step0 + ordinal * T;
-        const int k0 = ordinal * T;
-        const int w2 = 2 * nc;
-        auto load = [&](int r, XCell& c) {
-            c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
-            c.mat = mat[r * pitch + min(max(x, 0), nx)];  // unconditional (see load_cell)
-            if (!xin) return;
-            const int b = r * pitch + x;
-            c.ez = fin[b];
-            if (hok) {
-                c.hx = fin[2 * ps + b];
-                c.hy = fin[4 * ps + b];
-            }
-            const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
-            if (je >= 0) c.pex = pe_in[r * w2 + je];
-            if (jh >= 0) c.phx = ph_in[r * w2 + jh];
-        };
-        XCell cur[T];
-        float hxp[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = cur[t].pex = cur[t].phx = 0.f;
-            cur[t].mat = 0;
-            hxp[t] = 0.f;
-        }
-        auto store = [&](int yo, const XCell& c) {
-                const int b = yo * pitch + x;
-                fout[b] = c.ez;
-                if (hok) {
-                    fout[2 * ps + b] = c.hx;
-                    fout[4 * ps + b] = c.hy;
-                }
-                const int lo = span[3 * yo], hi = span[3 * yo + 1];
-                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
-                if (je >= 0) pe_out[yo * w2 + je] = c.pex;
-                if (jh >= 0) ph_out[yo * w2 + jh] = c.phx;
-        };
-        auto step = [&](int r, XCell in) {
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int y = r - 1 - t;
-                XCell c = cur[t];
-                if (t < nst) {
-                    const float dEx = lane_next(c.ez) - c.ez;
-                    const float dEy = in.ez - c.ez;
-                    c.phx = bxh * c.phx + axh * dEx;
-                    const float ty = dEx * kxh + dxf * c.phx;
-                    const float hy = hok ? c.hy + chy * ty : 0.f;
-                    const float hx = hok ? c.hx - chx * dEy : 0.f;
-                    const float dHy = hy - lane_prev(hy);
-                    const float dHx = hx - hxp[t];
-                    hxp[t] = hx;
-                    c.pex = bxe * c.pex + axe * dHy;
-                    const float cy = dHy * kxe + dxf * c.pex;
-                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                    float en = qa.x * c.ez + qa.y * cy - qa.y * dHx - qb.x * c.jp;
-                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
-                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
-                    c.ez = eok ? en : 0.f;
-                    c.hx = hx;
-                    c.hy = hy;
-                    if (wcc && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
-                        const int k = k0 + t;
-                        if (x == nx - D - 1) {
-                            samp[k * P + L1 + y - cy_lo] = c.hy;
-                            samp[(K + k) * P + L1 + y - cy_lo] = c.ez;
-                        } else if (x == nx - D) {
-                            samp[(2 * K + k) * P + L1 + y - cy_lo] = c.ez;
-                        } else if (x == D) {
-                            samp[k * P + L3 + (ny - D - 1 - y)] = c.hy;
-                            samp[(K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
-                        } else {
-                            samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
-                        }
-                    }
-                }
-                cur[t] = in;
-                in = c;
-            }
-            return in;
-        };
-        row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step);
+        if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        body_col<T>(tile, it, nst, Sp, ordinal, lc4, lcx);
     }
 }
+
 
 // Interior tiles of physical-mode simulations: every lane is inside the
 // grid, no CPML strip, NTFF contour row/column or PEC edge lies within the
@@ -953,13 +992,100 @@ struct ICell {
 };
 
 template <int T>
-__global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ tiles, int ntiles,
-                                                const LaunchItem* __restrict__ items,
-                                                const DevSim* __restrict__ sims, int ordinal) {
-    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+__device__ __forceinline__ void fill_inner(const DevSim* __restrict__ Sp, float4* __restrict__ lc4) {
+    {
+        float* lcf = (float*)lc4;
+        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchItem& it, int nst,
+                                         const DevSim* __restrict__ Sp, int ordinal,
+                                         const float4* __restrict__ lc4) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     constexpr int V = kWave - 2 * T;
+    const int par = it.parity0 ^ (ordinal & 1);
+    const int ps = Sp->ps, js = Sp->js, pitch = Sp->pitch;
+    gfloat* fin = (gfloat*)Sp->fbase + par * ps;
+    gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
+    gfloat* jin = (gfloat*)Sp->jbase + par * js;
+    gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
+    gcu8* mat = (gcu8*)Sp->mat;
+    kcint* span = (kcint*)Sp->span;
+    const float chx = Sp->chx, chy = Sp->chy;
+    const int x0 = tile.x0 + wv * V - T;
+    const int x = x0 + lane;
+    const bool own = lane >= T && lane < kWave - T;
+    const int sx = Sp->sx, sy = Sp->sy;
+    const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
+    kcfloat* srcw = (kcfloat*)Sp->srcw;
+    const int y0 = tile.y0, y1 = tile.y1;
+    const int step0 = it.step0 + ordinal * T;
+    auto load = [&](int r, ICell& c) {
+        const int b = r * pitch + x;
+        c.ez = fin[b];
+        c.hx = fin[2 * ps + b];
+        c.hy = fin[4 * ps + b];
+        c.mat = mat[b];
+        c.jp = 0.f;
+        const int lo = span[3 * r], hi = span[3 * r + 1];
+        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+    };
+    auto store = [&](int yo, const ICell& c) {
+        const int b = yo * pitch + x;
+        fout[b] = c.ez;
+        fout[2 * ps + b] = c.hx;
+        fout[4 * ps + b] = c.hy;
+        const int lo = span[3 * yo], hi = span[3 * yo + 1];
+        if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+    };
+    ICell cur[T];
+    float hxp[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = 0.f;
+        cur[t].mat = 0;
+        hxp[t] = 0.f;
+    }
+    auto step = [&](int r, ICell in) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int y = r - 1 - t;
+            ICell c = cur[t];
+            if (t < nst) {
+                // H half-step
+                const float dEx = lane_next(c.ez) - c.ez;
+                const float dEy = in.ez - c.ez;
+                const float hy = c.hy + chy * dEx;
+                const float hx = c.hx - chx * dEy;
+                // E half-step with the Debye polarisation current
+                const float dHy = hy - lane_prev(hy);
+                const float dHx = hx - hxp[t];
+                hxp[t] = hx;
+                const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
+                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                c.jp = qb.y * c.jp + qb.z * (en - c.ez);
+                c.ez = en;
+                c.hx = hx;
+                c.hy = hy;
+            }
+            cur[t] = in;
+            in = c;
+        }
+        return in;
+    };
+    row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step);
+
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ tiles, int ntiles,
+                                               const LaunchItem* __restrict__ items,
+                                               const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -967,84 +1093,42 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        {
-            float* lcf = (float*)lc4;
-            for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
-        }
+        fill_inner<T>(Sp, lc4);
         __syncthreads();
-        if (wv >= tile.nband) continue;  // after the last barrier of this tile
-        const int par = it.parity0 ^ (ordinal & 1);
-        const int ps = Sp->ps, js = Sp->js, pitch = Sp->pitch;
-        gfloat* fin = (gfloat*)Sp->fbase + par * ps;
-        gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
-        gfloat* jin = (gfloat*)Sp->jbase + par * js;
-        gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
-        gcu8* mat = (gcu8*)Sp->mat;
-        kcint* span = (kcint*)Sp->span;
-        const float chx = Sp->chx, chy = Sp->chy;
-        const int x0 = tile.x0 + wv * V - T;
-        const int x = x0 + lane;
-        const bool own = lane >= T && lane < kWave - T;
-        const int sx = Sp->sx, sy = Sp->sy;
-        const bool wsrc = (sx >= x0 && sx <= x0 + kWave - 1);
-        kcfloat* srcw = (kcfloat*)Sp->srcw;
-        const int y0 = tile.y0, y1 = tile.y1;
-        const int step0 = it.step0 + ordinal * T;
-        auto load = [&](int r, ICell& c) {
-            const int b = r * pitch + x;
-            c.ez = fin[b];
-            c.hx = fin[2 * ps + b];
-            c.hy = fin[4 * ps + b];
-            c.mat = mat[b];
-            c.jp = 0.f;
-            const int lo = span[3 * r], hi = span[3 * r + 1];
-            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
-        };
-        auto store = [&](int yo, const ICell& c) {
-            const int b = yo * pitch + x;
-            fout[b] = c.ez;
-            fout[2 * ps + b] = c.hx;
-            fout[4 * ps + b] = c.hy;
-            const int lo = span[3 * yo], hi = span[3 * yo + 1];
-            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
-        };
-        ICell cur[T];
-        float hxp[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            cur[t].ez = cur[t].hx = cur[t].hy = cur[t].jp = 0.f;
-            cur[t].mat = 0;
-            hxp[t] = 0.f;
-        }
-        auto step = [&](int r, ICell in) {
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int y = r - 1 - t;
-                ICell c = cur[t];
-                if (t < nst) {
-                    // H half-step
-                    const float dEx = lane_next(c.ez) - c.ez;
-                    const float dEy = in.ez - c.ez;
-                    const float hy = c.hy + chy * dEx;
-                    const float hx = c.hx - chx * dEy;
-                    // E half-step with the Debye polarisation current
-                    const float dHy = hy - lane_prev(hy);
-                    const float dHx = hx - hxp[t];
-                    hxp[t] = hx;
-                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                    float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
-                    if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
-                    c.jp = qb.y * c.jp + qb.z * (en - c.ez);
-                    c.ez = en;
-                    c.hx = hx;
-                    c.hy = hy;
-                }
-                cur[t] = in;
-                in = c;
-            }
-            return in;
-        };
-        row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step);
+        if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        body_inner<T>(tile, it, nst, Sp, ordinal, lc4);
+    }
+}
+
+
+// All four tile classes of physical-mode simulations in one launch (tile
+// class in TileItem::cls).  One launch per time block with no cross-stream
+// joins: the form for few simulations, where every block's latency is on the
+// critical path.  Classes are listed slowest first so their tiles start first.
+template <int T>
+__global__ __launch_bounds__(256) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
+                                                const LaunchItem* __restrict__ items,
+                                                const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float ltab[8 * TOMO_MAX_NC];
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;
+        const DevSim* Sp = sims + it.sim;
+        const int cls = tile.cls;
+        __syncthreads();
+        if (cls == TILE_CORNER) fill_edge<T>(Sp, lc4, ltab);
+        else if (cls == TILE_ROW) fill_row<T>(Sp, lc4, ltab);
+        else if (cls == TILE_COL) fill_col<T>(Sp, lc4, ltab);
+        else fill_inner<T>(Sp, lc4);
+        __syncthreads();
+        if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        if (cls == TILE_CORNER) body_edge<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else if (cls == TILE_ROW) body_row<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else if (cls == TILE_COL) body_col<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else body_inner<T>(tile, it, nst, Sp, ordinal, lc4);
     }
 }
 
@@ -1375,6 +1459,19 @@ hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchI
     }
     TI(1) TI(2) TI(4) TI(6) TI(8)
 #undef TI
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
+                         int ordinal, int waves, int grid, hipStream_t st) {
+    dim3 g(grid), b(waves * kWave);
+#define TF(TT)                                                                                       \
+    if (T == TT) {                                                                                   \
+        hipLaunchKernelGGL((k_tfused<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
+        return hipGetLastError();                                                                    \
+    }
+    TF(1) TF(2) TF(4) TF(6) TF(8)
+#undef TF
     return hipErrorInvalidValue;
 }
 

@@ -100,6 +100,7 @@ struct tomo_ctx {
     uint64_t budget = 0, used = 0;
     int T = 4, S = 32, W = 4, K = 32;  // schedule (W: waves per edge workgroup)
     int grid_mult = 4;                 // persistent workgroups per CU
+    int form = TOMO_FORM_STREAMS;      // launch form of physical-mode blocks
     bool use_graph = true;
     // DevSim table
     DevSim* d_sims = nullptr;
@@ -402,6 +403,7 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<Ti
             t.nband = (t.xe - x + V - 1) / V;
             t.y0 = y0;
             t.y1 = y1;
+            t.cls = k;  // TILE_CORNER, TILE_ROW, TILE_COL, TILE_INNER
             v.push_back(t);
             account(k, t);
         }
@@ -553,7 +555,8 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             const int cnt[3] = {n0, nr, ncl};
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
-            const bool multi = mode == TOMO_MODE_PHYSICAL && n_inner > 0 && nside > 0;
+            const bool fused = mode == TOMO_MODE_PHYSICAL && c->form == TOMO_FORM_FUSED;
+            const bool multi = mode == TOMO_MODE_PHYSICAL && !fused && n_inner > 0 && nside > 0;
             // per-class cell-steps and bytes of block o (kernel timing only)
             auto block_work = [&](int o, int k, double& cells, double& bytes) {
                 cells = bytes = 0;
@@ -580,6 +583,15 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
                                          std::min(n_edge, cap), c->stream));
                     lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
+                    continue;
+                }
+                if (fused) {
+                    const int nt = c->ntiles;
+                    LaunchTimer lt(c, c->stream);
+                    HIPCHK(launch_tfused(T, c->d_tiles, nt, c->d_items, c->d_sims, o, std::max(W, c->Wi),
+                                         std::min(nt, 2 * cap), c->stream));
+                    lt.done(TOMO_KERNEL_FUSED, cells[0] + cells[1] + cells[2] + cells[3],
+                            bytes[0] + bytes[1] + bytes[2] + bytes[3]);
                     continue;
                 }
                 // every class of block o starts after all of block o-1 (ev_bar)
@@ -1021,10 +1033,12 @@ static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
     const int T = s.tsteps ? s.tsteps : c->T, S = s.seg_rows ? s.seg_rows : c->S;
     const int W = s.edge_waves ? s.edge_waves : c->W, Wi = s.inner_waves ? s.inner_waves : c->Wi;
     const int K = s.chunk ? s.chunk : c->K, G = s.grid_mult ? s.grid_mult : c->grid_mult;
+    const int F = s.form ? s.form : c->form;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
-    if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64)
+    if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64 ||
+        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED))
         return fail(TOMO_E_INVALID, "bad schedule");
-    c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G;
+    c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F;
     return TOMO_OK;
 }
 
@@ -1040,7 +1054,7 @@ int tomo_set_schedule(tomo_ctx* c, const tomo_schedule* s) {
 int tomo_get_schedule(tomo_ctx* c, tomo_schedule* s) {
     if (!c || !s) return fail(TOMO_E_INVALID, "null argument");
     s->tsteps = c->T; s->seg_rows = c->S; s->edge_waves = c->W; s->inner_waves = c->Wi;
-    s->chunk = c->K; s->grid_mult = c->grid_mult;
+    s->chunk = c->K; s->grid_mult = c->grid_mult; s->form = c->form;
     return TOMO_OK;
 }
 
@@ -1090,7 +1104,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     }
     const bool prof = c->prof;
     c->prof = false;
-    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult};
+    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form};
     double cells = 0;
     for (Slot* s : sl) cells += (double)s->d.nx * s->d.ny * c->K;
     double best_t = 1e30;
@@ -1105,7 +1119,10 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     };
     if (!rc) rc = trial(cur);
     const int Ts[] = {1, 2, 4, 8}, Ss[] = {8, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
+    const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
+        if (tc.mode == TOMO_MODE_PHYSICAL)
+            for (int v : Fs) { tomo_schedule x = cur; x.form = v; if (!rc) rc = trial(x); }
         for (int v : Ts) { tomo_schedule x = cur; x.tsteps = v; if (!rc && c->K % v == 0) rc = trial(x); }
         for (int v : Ss) { tomo_schedule x = cur; x.seg_rows = v; if (!rc) rc = trial(x); }
         for (int v : Ws) { tomo_schedule x = cur; x.edge_waves = v; if (!rc) rc = trial(x); }

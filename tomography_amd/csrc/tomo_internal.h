@@ -80,6 +80,9 @@ struct LaunchItem {
     int nsteps;    // steps to do in this epoch (<= chunk)
 };
 
+// Tile classes of physical-mode simulations (TileItem::cls).
+enum { TILE_CORNER = 0, TILE_ROW = 1, TILE_COL = 2, TILE_INNER = 3 };
+
 // Work unit of the time-blocked update: `nband` (<= waves) adjacent column
 // bands over output rows [y0, y1) of one simulation.  Band k owns columns
 // [x0 + k V, min(x0 + (k+1) V, xe)) with V = 64 - 2T and reads T ghost
@@ -92,7 +95,8 @@ struct TileItem {
     int nband;     // bands in the group
     int y0, y1;    // output rows
     int xe;        // end of the owned columns of the tile (exclusive)
-    int pad[2];
+    int cls;       // TILE_* (physical mode)
+    int pad;
 };
 
 // Projection work description per (simulation, frequency).
