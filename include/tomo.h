@@ -163,7 +163,7 @@ enum {
     TOMO_KERNEL_COL = 3,     /* k_tcol: left / right frame between the corners */
     TOMO_KERNEL_INNER = 4,   /* k_tinner: clean interior bands */
     TOMO_KERNEL_FOLD = 5,    /* k_fold: running-DFT fold of a chunk of samples */
-    TOMO_KERNEL_PROJECT = 6, /* k_project: far-field projection */
+    TOMO_KERNEL_PROJECT = 6, /* k_project_mfma: far-field projection on the matrix cores */
     TOMO_KERNEL_FITNESS = 7, /* k_fitness: FP64 fitness reduction */
     TOMO_KERNEL_FUSED = 8,   /* k_tfused: every tile class in one launch (TOMO_FORM_FUSED) */
     TOMO_KERNEL_COUNT = 9

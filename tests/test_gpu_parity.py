@@ -148,11 +148,26 @@ def test_ref_eval_fitness_vs_oracle(ctx, oracle):
 
 
 def test_phys_eval_fitness_vs_oracle(ctx, oracle):
+    """Physical mode, two conditionings.  Far from the target (a large
+    tumour elsewhere) the fitness itself is within FF_TOL of FP64.  Near the
+    target (D - T is 4e-4 of D: the small tumour barely moves the total far
+    field) the fitness -||D - T||^2 / n amplifies the far-field error by
+    ~2||T|| / ||D - T||, so the bar is the first-order propagation of the
+    far-field error of both D and T: |dfit| <= 2 ||D - T|| (||dD|| + ||dT||) / n,
+    with ||dT|| bounded like ||dD||."""
     cfg = phys_small(steps=400)
+    n = cfg.n_exc * cfg.n_freq
+    far = np.array([-8.0, 9.0, 6.0, 30.0, 45.0, 15.0, 0.5, 0.0])
+    fit, D, _ = ctx.eval(cfg, far, want_farfield=True)
+    fo, Do, _ = oracle.eval(cfg, far, fp64=True)
+    assert rel_l2(D, Do) < FF_TOL
+    assert abs(fit - fo) <= FF_TOL * abs(fo), (fit, fo)
     fit, D, _ = ctx.eval(cfg, PHYS_PARAMS, want_farfield=True)
     fo, Do, _ = oracle.eval(cfg, PHYS_PARAMS, fp64=True)
     assert rel_l2(D, Do) < FF_TOL
-    assert abs(fit - fo) <= FF_TOL * abs(fo)
+    dD = np.linalg.norm(D.astype(np.float64) - Do)
+    bound = 2 * np.sqrt(abs(fo) * n) * 2 * dD / n
+    assert abs(fit - fo) <= FF_TOL * abs(fo) + bound, (fit, fo, bound)
 
 
 def test_batch_equals_plain(ctx):
@@ -257,7 +272,7 @@ def test_kernel_stats_account_for_every_cell_step(form):
     assert sum(k["cell_steps"] for k in upd) == want
     assert st["update"]["cell_steps"] == want
     ran = ("k_tedge", "k_trow", "k_tcol", "k_tinner") if form == 1 else ("k_tfused",)
-    for k in ran + ("k_fold", "k_project", "k_fitness"):
+    for k in ran + ("k_fold", "k_project_mfma", "k_fitness"):
         assert st[k]["launches"] > 0 and st[k]["ms"] > 0, k
     for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")) - set(ran):
         assert st[k]["launches"] == 0, k

@@ -39,7 +39,7 @@ class Schedule(_c.Structure):
         return tuple(getattr(self, k) for k, _ in self._fields_)
 
 
-KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project", "k_fitness",
+KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project_mfma", "k_fitness",
                 "k_tfused")
 
 

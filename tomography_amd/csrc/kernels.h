@@ -31,9 +31,11 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
                        hipStream_t st);
 
-// Far-field projection of the running DFT: D and phase per (sim, frequency).
-hipError_t launch_project(const ProjItem* items, int nitems, const DevSim* sims, const float* const* pos,
-                          int n_angles, float eta0, float* D, float* Pph, hipStream_t st);
+// Far-field projection of a group of simulations that share grid, contour and
+// frequencies, on the matrix cores: D/P at [(sim * n_freq + f) * n_angles + a];
+// kinv[f] = {wavenumber, far-field normaliser}.
+hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const DevSim* sims, const float2* kinv,
+                               int n_angles, float eta0, float* D, float* Pph, hipStream_t st);
 
 // FP64 fitness reduction, one block per evaluation:
 // fit[e] = -sum_i (D[e*L+i] - tgt[i])^2 / norm  (+ optional wrapped phase term).

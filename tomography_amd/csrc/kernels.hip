@@ -4,7 +4,8 @@
 //               E_field_update + calculate_JandM's sampling, FDTD_GPU.cu:231-434)
 //   k_fold      running DFT of one chunk of surface samples (calculate_JandM's
 //               accumulation, FDTD_GPU.cu:231-280), Kahan FP32 or FP64
-//   k_project   far-field projection (N2FPostProcess, FDTD_GPU.cu:553-634)
+//   k_project_mfma  far-field projection on the matrix cores
+//               (N2FPostProcess, FDTD_GPU.cu:553-634)
 //   k_fitness   FP64 fitness reduction (FDTD_GPU.cu:1004-1018)
 //   k_raster_*  material map (FDTD_GPU.cu:718-743 / the seeded phantom)
 //
@@ -1133,9 +1134,9 @@ __global__ __launch_bounds__(256) void k_tfused(const TileItem* __restrict__ til
 }
 
 // ---------------------------------------------------------------- running DFT fold
-// Phase e^{-j w (step + t0) dt}: reseeded in FP64 at the chunk start, advanced
-// by an FP32 rotation inside the chunk (<= 64 steps), summed in FP32 over the
-// chunk and added to the long-running accumulator with Kahan compensation.
+// Phase e^{-j w (step + t0) dt}: evaluated in FP64 for every step of the chunk
+// (<= 64 steps) into an LDS table and rounded once to FP32; the chunk is summed
+// in FP32 and added to the long-running accumulator with Kahan compensation.
 template <bool FP64>
 __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ items,
                                               const DevSim* __restrict__ sims) {
@@ -1149,23 +1150,13 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
     __shared__ double2 dJ[FP64 ? TOMO_MAX_FREQ * 64 : 1];
     __shared__ double2 dRot[FP64 ? TOMO_MAX_FREQ : 1];
     if (!FP64) {
-        if (threadIdx.x < 2 * F) {
-            const int f = threadIdx.x >> 1, which = threadIdx.x & 1;
-            const double w = S->wdt[f];
-            const double th0 = -w * ((double)it.step0 + (which ? S->tM : S->tJ));
-            double s0, c0, sr, cr;
-            sincos(th0, &s0, &c0);
-            sincos(-w, &sr, &cr);
-            float zr = (float)c0, zi = (float)s0;
-            const float rr = (float)cr, ri = (float)sr;
-            float2* dst = which ? wM : wJ;
-            for (int k = 0; k < nk; ++k) {
-                dst[f * 64 + k] = make_float2(zr, zi);
-                const float nr = zr * rr - zi * ri;
-                const float ni = zr * ri + zi * rr;
-                zr = nr;
-                zi = ni;
-            }
+        // every phase of the chunk evaluated in FP64 and rounded once to FP32
+        // (a rotation recurrence would drift by an ulp per step)
+        for (int i = threadIdx.x; i < 2 * F * nk; i += blockDim.x) {
+            const int which = i / (F * nk), f = (i / nk) % F, k = i % nk;
+            double s0, c0;
+            sincos(-S->wdt[f] * ((double)it.step0 + k + (which ? S->tM : S->tJ)), &s0, &c0);
+            (which ? wM : wJ)[f * 64 + k] = make_float2((float)c0, (float)s0);
         }
     } else {
         // FP64 study mode: exact phases; the M phase is the J phase rotated by
@@ -1236,68 +1227,116 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------- far-field projection
-// N(phi) = sum_p -del_p J_p w,  L(phi) = -sin(phi) sum_{y faces} dx M_p w + cos(phi) sum_{x faces} dy M_p w,
-// w = exp(j k (x_p cos phi + y_p sin phi)),  D = k^2 |L + eta0 N|^2 / norm,  P = arg(L + eta0 N).
-constexpr int kProjChunk = 256;
+// Far-field projection on the matrix cores.  For one frequency the
+// projection of every simulation of a group (same grid, contour and
+// frequencies) is one complex matrix product
+//     C[(s, c), a] = sum_p X[(s, c), p] W[p, a],   W[p, a] = e^{jk(x_p cos phi_a + y_p sin phi_a)},
+// with four rows per simulation: c = 0 carries -del_p J_p (-> N), c = 1
+// del_p M_p on y faces (-> L_y), c = 2 del_p M_p on x faces (-> L_x), c = 3
+// zero.  Each wave owns 32 angles x kProjTiles*32 rows and walks the contour
+// two points at a time with v_mfma_f32_32x32x2f32, four products per complex
+// step (Re = XrWr - XiWi, Im = XrWi + XiWr).  Operand layouts of the 32x32x2
+// f32 product: A[i][k] from lane i + 32k, B[k][j] from lane j + 32k, and
+// accumulator register r of lane l holds C[8(r/4) + 4(l/32) + r%4][l%32], so a
+// lane ends up holding all four rows of simulations 2q + l/32 (q = r/4) at
+// angle l%32 and forms D and the phase without leaving its registers.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kProjTiles = 4;  // 32-row tiles (8 simulations each) per wave
 
-__global__ __launch_bounds__(128) void k_project(const ProjItem* __restrict__ items,
-                                                 const DevSim* __restrict__ sims, int A, float eta0,
-                                                 float* __restrict__ D, float* __restrict__ Pph) {
-    const ProjItem it = items[blockIdx.y];
-    const DevSim* S = sims + it.sim;
-    const int P = S->P;
-    const bool fp64 = (S->flags & TOMO_FLAG_DFT_FP64) != 0;
-    __shared__ float sx[kProjChunk], sy[kProjChunk], jr[kProjChunk], ji[kProjChunk], mr[kProjChunk],
-        mi[kProjChunk], wt[kProjChunk];
-    __shared__ int sleg[kProjChunk];
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_project_mfma(const int* __restrict__ sim_idx, int n_sims,
+                                                     const DevSim* __restrict__ sims,
+                                                     const float2* __restrict__ kinv, int A, float eta0,
+                                                     float* __restrict__ D, float* __restrict__ Pph) {
+    const int lane = threadIdx.x;
+    const int a0 = blockIdx.x * 32, s0 = blockIdx.y * (8 * kProjTiles), f = blockIdx.z;
+    const DevSim* G = sims + sim_idx[0];  // geometry shared by the group
+    const int P = G->P, L1 = G->L1, L2 = G->L2, L3 = G->L3, F = G->nfreq;
+    const bool fp64 = (G->flags & TOMO_FLAG_DFT_FP64) != 0;
+    const float2 ki = kinv[f];
+    const float k = ki.x;
+    const float dxf = G->dxf, dyf = G->dyf;
+    const float* __restrict__ px = G->pos;
+    const float* __restrict__ py = G->pos + P;
+    // B operand lane: angle a0 + lane%32, point p0 + lane/32
+    const int ab = a0 + (lane & 31);
     float cph = 0.f, sph = 0.f;
-    if (a < A) sincosf(6.283185307179586f * (float)a / (float)A, &sph, &cph);
-    float Nr = 0.f, Ni = 0.f, Lyr = 0.f, Lyi = 0.f, Lxr = 0.f, Lxi = 0.f;
-    const float k = it.k;
-    for (int p0 = 0; p0 < P; p0 += kProjChunk) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < kProjChunk; i += blockDim.x) {
-            const int p = p0 + i;
-            if (p < P) {
-                sx[i] = S->pos[p];
-                sy[i] = S->pos[P + p];
-                if (fp64) {
-                    const double2 vj = ((const double2*)S->accJ)[(size_t)it.f * P + p];
-                    const double2 vm = ((const double2*)S->accM)[(size_t)it.f * P + p];
-                    jr[i] = (float)vj.x; ji[i] = (float)vj.y; mr[i] = (float)vm.x; mi[i] = (float)vm.y;
-                } else {
-                    const float4 vj = ((const float4*)S->accJ)[(size_t)it.f * P + p];
-                    const float4 vm = ((const float4*)S->accM)[(size_t)it.f * P + p];
-                    jr[i] = vj.x; ji[i] = vj.y; mr[i] = vm.x; mi[i] = vm.y;
-                }
-                const int leg = p < S->L1 ? 0 : (p < S->L2 ? 1 : (p < S->L3 ? 2 : 3));
-                sleg[i] = (leg == 0 || leg == 2) ? 0 : 1;
-                wt[i] = (leg == 0 || leg == 2) ? S->dxf : S->dyf;
-            } else {
-                sx[i] = sy[i] = jr[i] = ji[i] = mr[i] = mi[i] = wt[i] = 0.f;
-                sleg[i] = 0;
-            }
-        }
-        __syncthreads();
-        const int n = min(kProjChunk, P - p0);
-        for (int i = 0; i < n; ++i) {
-            float s, c;
-            sincosf(k * (sx[i] * cph + sy[i] * sph), &s, &c);
-            const float del = wt[i];
-            // N += -del * J * w
-            Nr -= del * (jr[i] * c - ji[i] * s);
-            Ni -= del * (jr[i] * s + ji[i] * c);
-            const float tr = del * (mr[i] * c - mi[i] * s), tiv = del * (mr[i] * s + mi[i] * c);
-            if (sleg[i] == 0) { Lyr += tr; Lyi += tiv; }
-            else { Lxr += tr; Lxi += tiv; }
+    sincosf(6.283185307179586f * (float)ab / (float)A, &sph, &cph);
+    const bool a_ok = ab < A;
+    // A operand lane: row lane%32 of each tile -> simulation (lane%32)/4, row kind (lane%32)%4
+    const int kh = lane >> 5;
+    const int rowk = lane & 3;
+    const void* accJ[kProjTiles];
+    const void* accM[kProjTiles];
+#pragma unroll
+    for (int m = 0; m < kProjTiles; ++m) {
+        const int s = s0 + 8 * m + ((lane & 31) >> 2);
+        accJ[m] = accM[m] = nullptr;
+        if (s < n_sims) {
+            const DevSim* S = sims + sim_idx[s];
+            accJ[m] = S->accJ;
+            accM[m] = S->accM;
         }
     }
-    if (a < A) {
-        const float Lr = -sph * Lyr + cph * Lxr, Li = -sph * Lyi + cph * Lxi;
-        const float Er = Lr + eta0 * Nr, Ei = Li + eta0 * Ni;
-        D[it.out + a] = k * k * (Er * Er + Ei * Ei) * it.inv_norm;
-        if (Pph) Pph[it.out + a] = atan2f(Ei, Er);
+    f32x16 cre[kProjTiles], cim[kProjTiles];
+#pragma unroll
+    for (int m = 0; m < kProjTiles; ++m)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) cre[m][r] = cim[m][r] = 0.f;
+    for (int p0 = 0; p0 < P; p0 += 2) {
+        const int p = p0 + kh;
+        float wr = 0.f, wi = 0.f, del = 0.f;
+        bool ysf = false;
+        if (p < P) {
+            if (a_ok) {
+                float sn, cs;
+                sincosf(k * (px[p] * cph + py[p] * sph), &sn, &cs);
+                wr = cs;
+                wi = sn;
+            }
+            ysf = p < L1 || (p >= L2 && p < L3);
+            del = ysf ? dxf : dyf;
+        }
+#pragma unroll
+        for (int m = 0; m < kProjTiles; ++m) {
+            float xr = 0.f, xi = 0.f;
+            if (p < P && rowk != 3 && accJ[m]) {
+                const size_t o = (size_t)f * P + p;
+                float vr, vi;
+                if (rowk == 0) {
+                    if (fp64) { const double2 v = ((const double2*)accJ[m])[o]; vr = (float)v.x; vi = (float)v.y; }
+                    else { const float4 v = ((const float4*)accJ[m])[o]; vr = v.x; vi = v.y; }
+                    xr = -del * vr;
+                    xi = -del * vi;
+                } else if ((rowk == 1) == ysf) {
+                    if (fp64) { const double2 v = ((const double2*)accM[m])[o]; vr = (float)v.x; vi = (float)v.y; }
+                    else { const float4 v = ((const float4*)accM[m])[o]; vr = v.x; vi = v.y; }
+                    xr = del * vr;
+                    xi = del * vi;
+                }
+            }
+            cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr, wr, cre[m], 0, 0, 0);
+            cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(-xi, wi, cre[m], 0, 0, 0);
+            cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr, wi, cim[m], 0, 0, 0);
+            cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi, wr, cim[m], 0, 0, 0);
+        }
+    }
+    if (!a_ok) return;
+    const float inv = ki.y;
+#pragma unroll
+    for (int m = 0; m < kProjTiles; ++m) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int s = s0 + 8 * m + 2 * q + kh;
+            if (s >= n_sims) continue;
+            const float Nr = cre[m][4 * q], Ni = cim[m][4 * q];
+            const float Lyr = cre[m][4 * q + 1], Lyi = cim[m][4 * q + 1];
+            const float Lxr = cre[m][4 * q + 2], Lxi = cim[m][4 * q + 2];
+            const float Lr = -sph * Lyr + cph * Lxr, Li = -sph * Lyi + cph * Lxi;
+            const float Er = Lr + eta0 * Nr, Ei = Li + eta0 * Ni;
+            const size_t o = ((size_t)s * F + f) * A + ab;
+            D[o] = k * k * (Er * Er + Ei * Ei) * inv;
+            if (Pph) Pph[o] = atan2f(Ei, Er);
+        }
     }
 }
 
@@ -1475,19 +1514,18 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const DevSim* sims, const float2* kinv,
+                               int n_angles, float eta0, float* D, float* Pph, hipStream_t st) {
+    dim3 g((n_angles + 31) / 32, (n_sims + 8 * kProjTiles - 1) / (8 * kProjTiles), n_freq), b(64);
+    hipLaunchKernelGGL(k_project_mfma, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
+    return hipGetLastError();
+}
+
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
                        hipStream_t st) {
     dim3 g((max_p + 255) / 256, nitems), b(256);
     if (fp64) hipLaunchKernelGGL((k_fold<true>), g, b, 0, st, items, sims);
     else hipLaunchKernelGGL((k_fold<false>), g, b, 0, st, items, sims);
-    return hipGetLastError();
-}
-
-hipError_t launch_project(const ProjItem* items, int nitems, const DevSim* sims, const float* const* pos,
-                          int n_angles, float eta0, float* D, float* Pph, hipStream_t st) {
-    (void)pos;
-    dim3 g((n_angles + 127) / 128, nitems), b(128);
-    hipLaunchKernelGGL(k_project, g, b, 0, st, items, sims, n_angles, eta0, D, Pph);
     return hipGetLastError();
 }
 

@@ -664,34 +664,66 @@ static int check_uniform_precision(const std::vector<Slot*>& sl) {
 }
 
 // Project the far field of `sl` into D/P (device, [slot][freq][angle] at
-// offset out_off[i]).
+// offset out_off[i]).  Consecutive simulations that share grid, contour,
+// frequencies and angles and whose records are contiguous form one group: one
+// matrix-core launch per group.
 static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<int>& out_off, float* dD,
                    float* dP) {
     if (sl.empty()) return TOMO_OK;
-    std::vector<ProjItem> items;
-    int A = sl[0]->su.cfg.n_angles;
+    auto same = [](const tomo_sim_config& x, const tomo_sim_config& y) {
+        return x.mode == y.mode && x.nx == y.nx && x.ny == y.ny && x.ntff_offset == y.ntff_offset &&
+               x.dx == y.dx && x.dy == y.dy && x.n_freq == y.n_freq && x.n_angles == y.n_angles &&
+               x.f_start == y.f_start && x.f_stop == y.f_stop && x.pulse_tau == y.pulse_tau &&
+               x.pulse_t0 == y.pulse_t0 && x.ff_norm == y.ff_norm &&
+               ((x.flags ^ y.flags) & TOMO_FLAG_DFT_FP64) == 0;
+    };
+    struct Group { size_t i0, n; };
+    std::vector<Group> groups;
     for (size_t i = 0; i < sl.size(); ++i) {
-        const SimSetup& su = sl[i]->su;
-        if (su.cfg.n_angles != A) return fail(TOMO_E_INVALID, "mixed angle counts in one projection");
-        for (int f = 0; f < su.cfg.n_freq; ++f) {
-            ProjItem p{};
-            p.sim = sl[i]->idx;
-            p.f = f;
-            p.out = out_off[i] + f * A;
-            p.k = (float)(2 * kPI * su.freq[f] / c0());
-            p.inv_norm = (float)su.inv_norm[f];
-            items.push_back(p);
+        const tomo_sim_config& ci = sl[i]->su.cfg;
+        const int rec = ci.n_freq * ci.n_angles;
+        if (!groups.empty()) {
+            Group& g = groups.back();
+            const Slot* f = sl[g.i0];
+            if (same(f->su.cfg, ci) && out_off[i] == out_off[g.i0] + (int)(i - g.i0) * rec) {
+                ++g.n;
+                continue;
+            }
         }
+        groups.push_back({i, 1});
     }
-    ProjItem* d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(ProjItem) * items.size()));
-    HIPCHK(hipMemcpyAsync(d, items.data(), sizeof(ProjItem) * items.size(), hipMemcpyHostToDevice, c->stream));
-    LaunchTimer lt(c);
-    hipError_t e = launch_project(d, (int)items.size(), c->d_sims, nullptr, A, (float)eta0(), dD, dP, c->stream);
-    lt.done(TOMO_KERNEL_PROJECT);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipFree(d));
+    // staging: per group the simulation indices and per frequency {k, normaliser}
+    std::vector<int> idx(sl.size());
+    std::vector<float2> kinv;
+    std::vector<size_t> kofs;
+    for (size_t i = 0; i < sl.size(); ++i) idx[i] = sl[i]->idx;
+    for (const Group& g : groups) {
+        const SimSetup& su = sl[g.i0]->su;
+        kofs.push_back(kinv.size());
+        for (int f = 0; f < su.cfg.n_freq; ++f)
+            kinv.push_back(make_float2((float)(2 * kPI * su.freq[f] / c0()), (float)su.inv_norm[f]));
+    }
+    const size_t bytes = align256(idx.size() * sizeof(int)) + kinv.size() * sizeof(float2);
+    char* d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    int* d_idx = (int*)d;
+    float2* d_k = (float2*)(d + align256(idx.size() * sizeof(int)));
+    hipError_t e = hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_k, kinv.data(), kinv.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+    for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
+        const Group& g = groups[gi];
+        const tomo_sim_config& cf = sl[g.i0]->su.cfg;
+        LaunchTimer lt(c);
+        e = launch_project_mfma(d_idx + g.i0, (int)g.n, cf.n_freq, c->d_sims, d_k + kofs[gi], cf.n_angles,
+                                (float)eta0(), dD + out_off[g.i0], dP ? dP + out_off[g.i0] : nullptr, c->stream);
+        lt.done(TOMO_KERNEL_PROJECT);
+    }
+    // the uploads read host vectors of this frame
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
     HIPCHK(e);
+    HIPCHK(es);
     return TOMO_OK;
 }
 

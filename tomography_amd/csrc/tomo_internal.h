@@ -99,12 +99,3 @@ struct TileItem {
     int pad;
 };
 
-// Projection work description per (simulation, frequency).
-struct ProjItem {
-    int sim;       // DevSim index
-    int f;         // frequency index
-    int out;       // offset (in floats) of this item's D/P record
-    int pad;
-    float k;       // wavenumber
-    float inv_norm;
-};
