@@ -38,6 +38,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-steps/s & fitness evals/s, batched vs plain; p50/p99 eval latency"
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def seeded_vectors(n, dim, seed):
     """Parameter vectors inside the physical bounds (tumour centre, radius, Debye values)."""
     rng = np.random.default_rng(seed)
@@ -215,6 +220,101 @@ def run_serve(ctx, n_req, rate, seed=20261205):
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4)}
 
 
+def uneven_requests(n, seed=20261204):
+    """The uneven batch: grids log-uniform in [256, 2048] (steps clamp(8N, 2048,
+    16384), one antenna each) and 0-3 tumours per phantom (0: a zero-radius
+    tumour block)."""
+    rng = np.random.default_rng(seed)
+    grids = np.exp(rng.uniform(np.log(256), np.log(2048), n)).astype(int)
+    ntum = rng.integers(0, 4, n)
+    reqs = []
+    for i in range(n):
+        cfg = t.preset(t.PRESET_CFG4, int(grids[i]))
+        v = seeded_vectors(1, 8 * max(1, int(ntum[i])), seed + 1 + i)[0]
+        if ntum[i] == 0:
+            v[2] = 0.0
+        reqs.append((cfg, v))
+    return reqs
+
+
+def run_uneven(ctx, n, plain_sample, batch_sched, plain_sched):
+    """All n requests submitted at once and run to completion through the
+    request loop (continuous batching over mixed grids), against the plain
+    path (tomo_eval, one simulation per launch sequence) on a sample."""
+    reqs = uneven_requests(n)
+    ctx.set_schedule(**batch_sched)
+    for g in sorted({r[0].nx for r in reqs}):  # targets are set-up, not timed
+        ctx.target_farfield(t.preset(t.PRESET_CFG4, g))
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    tickets = [ctx.submit(c, v) for c, v in reqs]
+    lat, pending = {}, set(tickets)
+    last = time.perf_counter()
+    while pending:
+        ctx.progress()
+        if time.perf_counter() - last > 10:
+            log(f"uneven: {n - len(pending)}/{n} done")
+            last = time.perf_counter()
+        for tk in list(pending):
+            ok, fit, l = ctx.poll(tk)
+            if ok:
+                lat[tk] = l
+                pending.discard(tk)
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.profile(False)
+    cells = sum(cell_steps(c) for c, _ in reqs)
+    out = {"workload": f"{n} requests, grids log-uniform 256^2..2048^2, steps clamp(8N, 2048, 16384), 1 antenna, "
+                       "0-3 tumours (CFG4)", "cell_steps": cells, "wall_s": round(wall, 3),
+           "Mcell_steps_per_s": round(cells / wall / 1e6, 1), "evals_per_s": round(n / wall, 3),
+           "p50_latency_s": round(float(np.percentile(list(lat.values()), 50)), 3),
+           "p99_latency_s": round(float(np.percentile(list(lat.values()), 99)), 3),
+           "update_phase_Mcell_steps_per_s": round(st["update"]["cell_steps"] / (st["update"]["ms"] / 1e3) / 1e6, 1)
+           if st["update"]["ms"] else None}
+    if plain_sample:
+        ctx.set_schedule(**plain_sched)
+        idx = np.random.default_rng(7).choice(n, plain_sample, replace=False)
+        tp, cp = 0.0, 0.0
+        for i in idx:
+            c, v = reqs[i]
+            t1 = time.perf_counter()
+            ctx.eval(c, v)
+            tp += time.perf_counter() - t1
+            cp += cell_steps(c)
+        # plain evals/s on this mix: the sample's mean seconds per eval
+        out["plain_sample"] = {"evals": int(plain_sample), "s": round(tp, 3), "evals_per_s": round(plain_sample / tp, 3),
+                               "Mcell_steps_per_s": round(cp / tp / 1e6, 1)}
+        out["batched_vs_plain_evals_per_s"] = round(out["evals_per_s"] / out["plain_sample"]["evals_per_s"], 3)
+    return out
+
+
+def run_precision(ctx, n, max_grid=320, oracle_grid=288):
+    """Per-result error of the Kahan-FP32 running DFT and of the FP64 one
+    (both with FP32 fields) against the FP64 oracle, on the smallest grids of
+    the uneven set: relative L2 of the far field and relative fitness error."""
+    from oracle.binding import Oracle
+    o = Oracle()
+    reqs = [r for r in uneven_requests(512) if r[0].nx <= max_grid][:n]
+    rows = []
+    for c, v in reqs:
+        log(f"precision: grid {c.nx}, {c.steps} steps")
+        row = {"grid": c.nx, "steps": c.steps, "tumour_blocks": len(v) // 8}
+        f32, d32, _ = ctx.eval(c, v, want_farfield=True)
+        c64 = c.copy(flags=c.flags | t.FLAG_DFT_FP64)
+        f64, d64, _ = ctx.eval(c64, v, want_farfield=True)
+        row["rel_D_kahan_vs_fp64dft"] = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+        if c.nx <= oracle_grid:
+            fo, do, _ = o.eval(c, v, fp64=True)
+            row["rel_D_kahan_vs_oracle"] = float(np.linalg.norm(d32 - do) / np.linalg.norm(do))
+            row["rel_D_fp64dft_vs_oracle"] = float(np.linalg.norm(d64 - do) / np.linalg.norm(do))
+            row["rel_fit_kahan_vs_oracle"] = float(abs(f32 - fo) / abs(fo))
+            row["rel_fit_fp64dft_vs_oracle"] = float(abs(f64 - fo) / abs(fo))
+        rows.append(row)
+    keys = [k for k in rows[0] if k.startswith("rel_")] if rows else []
+    summary = {k: float(max(r[k] for r in rows if k in r)) for k in keys if any(k in r for r in rows)}
+    return {"results": rows, "max": summary}
+
+
 def cpu_baseline(steps=2000):
     """The oracle's FP32 restatement on one host core, first `steps` steps of
     the 1024^2 plain workload (fields, CPML, Debye, running DFT)."""
@@ -275,6 +375,9 @@ def main():
     ap.add_argument("--plain-evals", type=int, default=2,
                     help="plain-path evals on the batched workload (for the batched/plain ratio; 0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
+    ap.add_argument("--uneven", type=int, default=0, help="requests of the uneven-batch study (512 in its definition)")
+    ap.add_argument("--uneven-plain-sample", type=int, default=8, help="plain-path evals timed on the uneven mix")
+    ap.add_argument("--precision", type=int, default=0, help="results of the FP32-Kahan vs FP64 precision study")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -302,10 +405,10 @@ def main():
         rl["traffic"] = traffic_for("plain_cfg2", rl["kernel"], tuple(sched.values()))
     out["roofline"] = rl
     out["kernel_ms_per_step"] = kernel_ms(st, args.steps)
+    bsch = dict(BATCH_SCHEDULE)
+    if args.batch_schedule:
+        bsch = dict(zip(SCHED_KEYS, [int(x) for x in args.batch_schedule.split(",")]))
     if not args.no_batched:
-        bsch = dict(BATCH_SCHEDULE)
-        if args.batch_schedule:
-            bsch = dict(zip(SCHED_KEYS, [int(x) for x in args.batch_schedule.split(",")]))
         bcfg, bdt, bst, nvec, bfits, bsched = run_batched(ctx, bsch, tune)
         bcs = cell_steps(bcfg) * nvec
         brl = roofline_from(bst)
@@ -320,6 +423,12 @@ def main():
             pl = run_plain_same_workload(ctx, args.plain_evals, sched, bfits)
             out["batched"]["plain_same_workload"] = pl
             out["batched"]["batched_vs_plain_evals_per_s"] = round(out["batched"]["evals_per_s"] / pl["evals_per_s"], 3)
+    if args.serve:
+        ctx.set_schedule(**bsch)  # the many-simulation schedule for the request loop
+    if args.uneven:
+        out["uneven"] = run_uneven(ctx, args.uneven, args.uneven_plain_sample, bsch, sched)
+    if args.precision:
+        out["precision"] = run_precision(ctx, args.precision)
     if args.serve:
         out["serving"] = run_serve(ctx, args.serve, args.rate)
     if args.calib_copy_mb:

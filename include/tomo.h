@@ -227,6 +227,39 @@ int tomo_progress(tomo_ctx *ctx, int *n_active_out);
 /* Resident device bytes one request of this configuration needs. */
 int64_t tomo_request_bytes(const tomo_sim_config *cfg);
 
+/* ---- asynchronous search drivers ---------------------------------------- */
+/*
+ * Population search over the request loop, replacing the reference's TAO
+ * master/worker drivers (test.cxx:79-121: ParticleSwarmMPI / DifferentialEvolutionMPI
+ * calling FDTD_GPU through MPI workers).  Every evaluation is a tomo_submit
+ * request; up to `inflight` are resident at once and a new one is issued as
+ * soon as a result returns (asynchronous, steady state).
+ *   TOMO_SEARCH_DE  : DE/rand/1/bin, differential weight F, crossover rate CR;
+ *                     a trial replaces its member when its fitness is >=.
+ *   TOMO_SEARCH_PSO : particle swarm with constriction (chi 0.7298, c1 = c2 = 1.49618).
+ * Physical-mode configurations only (the request loop serves physical mode).
+ * best_x receives the best vector (dim values).
+ */
+#define TOMO_SEARCH_DE 1
+#define TOMO_SEARCH_PSO 2
+typedef struct tomo_search_config {
+    int32_t method;    /* TOMO_SEARCH_* */
+    int32_t pop;       /* population size (>= 4) */
+    int32_t dim;       /* parameter-vector length (>= 8) */
+    int32_t max_evals; /* evaluations to run (>= pop) */
+    int32_t inflight;  /* evaluations resident at once (0 = pop) */
+    int32_t pad;
+    double F, CR;      /* DE weight and crossover rate (0 = 0.5, 0.9) */
+    uint64_t seed;
+} tomo_search_config;
+typedef struct tomo_search_result {
+    double best_fitness;
+    int64_t evals, failed;
+    double wall_s, evals_per_s, p50_latency_s, p99_latency_s;
+} tomo_search_result;
+int tomo_search_run(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *lo, const double *hi,
+                    const tomo_search_config *sc, double *best_x, tomo_search_result *res);
+
 /* ---- single-simulation access (parity tests) --------------------------- */
 typedef struct tomo_sim tomo_sim;
 /* Build one simulation of excitation/antenna `exc` of the phantom `params`. */

@@ -93,11 +93,12 @@ def test_header_constants_and_struct_layouts_match_the_python_mirror(tmp_path):
     assert len(names) == int(defs["KERNEL_COUNT"])
     assert names[int(defs["KERNEL_FUSED"])] == "k_tfused" and names[int(defs["KERNEL_INNER"])] == "k_tinner"
     src = tmp_path / "sizes.c"
-    src.write_text('#include <stdio.h>\n#include "tomo.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
-                   'sizeof(tomo_sim_config), sizeof(tomo_schedule), sizeof(tomo_stats), sizeof(tomo_kernel_stats));'
-                   'return 0;}\n')
+    src.write_text('#include <stdio.h>\n#include "tomo.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
+                   'sizeof(tomo_sim_config), sizeof(tomo_schedule), sizeof(tomo_stats), sizeof(tomo_kernel_stats),'
+                   'sizeof(tomo_search_config), sizeof(tomo_search_result));return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True).stdout.split()]
     assert got == [ctypes.sizeof(t.SimConfig), ctypes.sizeof(t.api.Schedule), ctypes.sizeof(t.api.Stats),
-                   ctypes.sizeof(t.api.KernelStats)]
+                   ctypes.sizeof(t.api.KernelStats), ctypes.sizeof(t.api.SearchConfig),
+                   ctypes.sizeof(t.api.SearchResult)]

@@ -279,3 +279,18 @@ def test_kernel_stats_account_for_every_cell_step(form):
     b = sum(k["alg_bytes"] for k in upd) / want
     assert 25.0 < b < 49.0
     c.close()
+
+
+@pytest.mark.parametrize("method", [t.SEARCH_DE, t.SEARCH_PSO])
+def test_search_driver(ctx, method):
+    """The asynchronous DE / PSO drivers run every evaluation through the
+    request loop; the best vector they report stays inside the bounds and
+    evaluates (plain path) to exactly the best fitness they report."""
+    cfg = phys_small(steps=160, n=128, n_exc=1)
+    lo = np.array([-10.0, -10.0, 2.0, 10.0, 10.0, 8.0, 0.0, 0.0])
+    hi = np.array([10.0, 10.0, 6.0, 30.0, 50.0, 20.0, 1.0, 0.0])
+    best, res = ctx.search(cfg, lo, hi, method, pop=8, max_evals=32, inflight=8, seed=3)
+    assert res["evals"] == 32 and res["failed"] == 0
+    assert np.all(best >= lo) and np.all(best <= hi)
+    assert ctx.eval(cfg, best) == res["best_fitness"]
+    assert res["p99_latency_s"] >= res["p50_latency_s"] > 0

@@ -35,6 +35,9 @@ PRESET_CFG5 = 6
 FORM_STREAMS = 1
 FORM_FUSED = 2
 
+SEARCH_DE = 1
+SEARCH_PSO = 2
+
 
 class SimConfig(_c.Structure):
     """tomo_sim_config (include/tomo.h)."""

@@ -25,6 +25,7 @@ EXPORTS = [
     "tomo_target_farfield", "tomo_submit", "tomo_poll", "tomo_wait", "tomo_cancel", "tomo_progress",
     "tomo_request_bytes", "tomo_sim_create", "tomo_sim_destroy", "tomo_sim_step", "tomo_sim_fields",
     "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_get_stats",
+    "tomo_search_run",
 ]
 
 
@@ -63,6 +64,25 @@ class Stats(_c.Structure):
         d = {name: self.kernel[i].as_dict() for i, name in enumerate(KERNEL_NAMES)}
         d["update"] = self.update.as_dict()
         return d
+
+
+class SearchConfig(_c.Structure):
+    """tomo_search_config (include/tomo.h)."""
+
+    _fields_ = [("method", _c.c_int32), ("pop", _c.c_int32), ("dim", _c.c_int32), ("max_evals", _c.c_int32),
+                ("inflight", _c.c_int32), ("pad", _c.c_int32), ("F", _c.c_double), ("CR", _c.c_double),
+                ("seed", _c.c_uint64)]
+
+
+class SearchResult(_c.Structure):
+    """tomo_search_result (include/tomo.h)."""
+
+    _fields_ = [("best_fitness", _c.c_double), ("evals", _c.c_int64), ("failed", _c.c_int64),
+                ("wall_s", _c.c_double), ("evals_per_s", _c.c_double), ("p50_latency_s", _c.c_double),
+                ("p99_latency_s", _c.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TomoError(RuntimeError):
@@ -110,6 +130,7 @@ def lib():
     L.tomo_sim_farfield.argtypes = [vp, _fp, _fp]
     L.tomo_profile.argtypes = [vp, _c.c_int]
     L.tomo_get_stats.argtypes = [vp, _c.c_void_p]
+    L.tomo_search_run.argtypes = [vp, vp, _dp, _dp, vp, _dp, vp]
     _lib = L
     return L
 
@@ -173,6 +194,17 @@ class Context:
         st = Stats()
         _check(lib().tomo_get_stats(self.h, _c.byref(st)))
         return st.as_dict()
+
+    def search(self, cfg, lo, hi, method, pop, max_evals, inflight=0, F=0.5, CR=0.9, seed=1):
+        """Asynchronous DE / PSO over the request loop: returns (best_x, result dict)."""
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        sc = SearchConfig(method, pop, lo.size, max_evals, inflight, 0, F, CR, seed)
+        best = np.zeros(lo.size)
+        res = SearchResult()
+        _check(lib().tomo_search_run(self.h, _c.byref(cfg), _p(lo, _dp), _p(hi, _dp), _c.byref(sc), _p(best, _dp),
+                                     _c.byref(res)))
+        return best, res.as_dict()
 
     def schedule(self):
         """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form)"""

@@ -20,7 +20,7 @@ ARCH = os.environ.get("TOMO_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
-SOURCES = ["kernels.hip", "setup.cpp", "runtime.cpp", "fdtd_gpu_shim.cpp"]
+SOURCES = ["kernels.hip", "setup.cpp", "runtime.cpp", "search.cpp", "fdtd_gpu_shim.cpp"]
 
 
 def _run(cmd):
