@@ -1137,6 +1137,8 @@ __global__ __launch_bounds__(256) void k_tfused(const TileItem* __restrict__ til
 // Phase e^{-j w (step + t0) dt}: evaluated in FP64 for every step of the chunk
 // (<= 64 steps) into an LDS table and rounded once to FP32; the chunk is summed
 // in FP32 and added to the long-running accumulator with Kahan compensation.
+constexpr int kFoldGroup = 8;  // frequencies per pass of the fold
+
 template <bool FP64>
 __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ items,
                                               const DevSim* __restrict__ sims) {
@@ -1181,47 +1183,67 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
     const float* hs = S->samp + p;
     const float* e0 = S->samp + (size_t)K * P + p;
     const float* e1 = S->samp + (size_t)2 * K * P + p;
-    for (int f = 0; f < F; ++f) {
+    // Frequencies in groups of kFoldGroup: each sample is read once per group
+    // (coalesced over p) and multiplied into every frequency of the group.
+    for (int f0 = 0; f0 < F; f0 += kFoldGroup) {
         if (!FP64) {
-            float jr = 0.f, ji = 0.f, mr = 0.f, mi = 0.f;
+            float jr[kFoldGroup], ji[kFoldGroup], mr[kFoldGroup], mi[kFoldGroup];
+#pragma unroll
+            for (int q = 0; q < kFoldGroup; ++q) jr[q] = ji[q] = mr[q] = mi[q] = 0.f;
             for (int k = 0; k < nk; ++k) {
                 const float h = hs[(size_t)k * P];
                 const float e = (e0[(size_t)k * P] + e1[(size_t)k * P]) / 2;
-                const float2 a = wJ[f * 64 + k], b = wM[f * 64 + k];
-                jr += h * a.x;
-                ji += h * a.y;
-                mr += e * b.x;
-                mi += e * b.y;
+#pragma unroll
+                for (int q = 0; q < kFoldGroup; ++q) {
+                    const float2 a = wJ[(f0 + q) * 64 + k], b = wM[(f0 + q) * 64 + k];
+                    jr[q] += h * a.x;
+                    ji[q] += h * a.y;
+                    mr[q] += e * b.x;
+                    mi[q] += e * b.y;
+                }
             }
-            float4* aJ = (float4*)S->accJ + (size_t)f * P + p;
-            float4* aM = (float4*)S->accM + (size_t)f * P + p;
-            float4 vJ = *aJ, vM = *aM;
-            // Kahan: value (x, y), compensation (z, w)
-            float yv, tv;
-            yv = sdt * jr - vJ.z; tv = vJ.x + yv; vJ.z = (tv - vJ.x) - yv; vJ.x = tv;
-            yv = sdt * ji - vJ.w; tv = vJ.y + yv; vJ.w = (tv - vJ.y) - yv; vJ.y = tv;
-            yv = sdt * mr - vM.z; tv = vM.x + yv; vM.z = (tv - vM.x) - yv; vM.x = tv;
-            yv = sdt * mi - vM.w; tv = vM.y + yv; vM.w = (tv - vM.y) - yv; vM.y = tv;
-            *aJ = vJ;
-            *aM = vM;
+#pragma unroll
+            for (int q = 0; q < kFoldGroup; ++q) {
+                if (f0 + q >= F) break;
+                float4* aJ = (float4*)S->accJ + (size_t)(f0 + q) * P + p;
+                float4* aM = (float4*)S->accM + (size_t)(f0 + q) * P + p;
+                float4 vJ = *aJ, vM = *aM;
+                // Kahan: value (x, y), compensation (z, w)
+                float yv, tv;
+                yv = sdt * jr[q] - vJ.z; tv = vJ.x + yv; vJ.z = (tv - vJ.x) - yv; vJ.x = tv;
+                yv = sdt * ji[q] - vJ.w; tv = vJ.y + yv; vJ.w = (tv - vJ.y) - yv; vJ.y = tv;
+                yv = sdt * mr[q] - vM.z; tv = vM.x + yv; vM.z = (tv - vM.x) - yv; vM.x = tv;
+                yv = sdt * mi[q] - vM.w; tv = vM.y + yv; vM.w = (tv - vM.y) - yv; vM.y = tv;
+                *aJ = vJ;
+                *aM = vM;
+            }
         } else {
-            double jr = 0, ji = 0, mr = 0, mi = 0;
+            double jr[kFoldGroup], ji[kFoldGroup], mr[kFoldGroup], mi[kFoldGroup];
+#pragma unroll
+            for (int q = 0; q < kFoldGroup; ++q) jr[q] = ji[q] = mr[q] = mi[q] = 0.0;
             for (int k = 0; k < nk; ++k) {
                 const double h = hs[(size_t)k * P];
                 const double e = 0.5 * ((double)e0[(size_t)k * P] + (double)e1[(size_t)k * P]);
-                const double2 a = dJ[f * 64 + k], r = dRot[f];
-                const double2 b = make_double2(a.x * r.x - a.y * r.y, a.x * r.y + a.y * r.x);
-                jr += h * a.x;
-                ji += h * a.y;
-                mr += e * b.x;
-                mi += e * b.y;
+#pragma unroll
+                for (int q = 0; q < kFoldGroup; ++q) {
+                    const double2 a = dJ[(f0 + q) * 64 + k], r = dRot[f0 + q];
+                    const double2 b = make_double2(a.x * r.x - a.y * r.y, a.x * r.y + a.y * r.x);
+                    jr[q] += h * a.x;
+                    ji[q] += h * a.y;
+                    mr[q] += e * b.x;
+                    mi[q] += e * b.y;
+                }
             }
-            double2* aJ = (double2*)S->accJ + (size_t)f * P + p;
-            double2* aM = (double2*)S->accM + (size_t)f * P + p;
-            double2 vJ = *aJ, vM = *aM;
-            vJ.x += sdt * jr; vJ.y += sdt * ji; vM.x += sdt * mr; vM.y += sdt * mi;
-            *aJ = vJ;
-            *aM = vM;
+#pragma unroll
+            for (int q = 0; q < kFoldGroup; ++q) {
+                if (f0 + q >= F) break;
+                double2* aJ = (double2*)S->accJ + (size_t)(f0 + q) * P + p;
+                double2* aM = (double2*)S->accM + (size_t)(f0 + q) * P + p;
+                double2 vJ = *aJ, vM = *aM;
+                vJ.x += sdt * jr[q]; vJ.y += sdt * ji[q]; vM.x += sdt * mr[q]; vM.y += sdt * mi[q];
+                *aJ = vJ;
+                *aM = vM;
+            }
         }
     }
 }
