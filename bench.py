@@ -182,42 +182,96 @@ def run_plain_same_workload(ctx, n, sched, batch_fits):
             "Mcell_steps_per_s": round(cell_steps(cfg) * n / dt / 1e6, 1), "max_rel_fitness_diff_vs_batched": rel}
 
 
-def run_serve(ctx, n_req, rate, seed=20261205):
-    """Poisson arrivals of mixed requests (grids 256^2..1024^2, 1-4 antennas)."""
+SERVE_GRIDS = (256, 362, 512, 724, 1024, 1448, 2048)  # log-spaced 256^2..2048^2
+SERVE_ANTENNAS = (1, 2, 4, 8, 16)
+
+
+def serve_requests(n_req, seed=20261205):
+    """Serving mix: grids log-spaced 256^2..2048^2, 1-16 antennas, 8-32-dim
+    vectors (1-4 tumour blocks), drawn per request."""
     rng = np.random.default_rng(seed)
-    grids = np.exp(rng.uniform(np.log(256), np.log(1024), n_req)).astype(int) // 32 * 32
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     reqs = []
     for i in range(n_req):
-        cfg = t.preset(t.PRESET_CFG5, int(grids[i])).copy(n_exc=int(rng.integers(1, 5)))
+        g = int(rng.choice(SERVE_GRIDS))
+        cfg = t.preset(t.PRESET_CFG5, g).copy(n_exc=int(rng.choice(SERVE_ANTENNAS)))
         dim = int(rng.choice([8, 16, 24, 32]))
-        reqs.append((arrivals[i], cfg, seeded_vectors(1, dim, seed + i)[0]))
-    for g in sorted(set(int(x) for x in grids)):  # targets are part of the set-up, not the timed loop
-        for ne in range(1, 5):
-            ctx.target_farfield(t.preset(t.PRESET_CFG5, g).copy(n_exc=ne))
-    t0 = time.perf_counter()
-    pending, lat, done, i = {}, [], 0, 0
-    while done < n_req:
-        now = time.perf_counter() - t0
-        while i < n_req and reqs[i][0] <= now:
-            pending[ctx.submit(reqs[i][1], reqs[i][2])] = True
-            i += 1
-        if not pending:
-            time.sleep(max(0.0, min(0.01, reqs[i][0] - now)))
-            continue
-        ctx.progress()
-        for tk in list(pending):
-            ok, fit, l = ctx.poll(tk)
-            if ok:
-                lat.append(l)
-                del pending[tk]
-                done += 1
-    wall = time.perf_counter() - t0
-    cells = sum(cell_steps(r[1]) for r in reqs)
-    return {"requests": n_req, "rate_per_s": rate, "wall_s": round(wall, 3),
-            "evals_per_s": round(n_req / wall, 3), "Mcell_steps_per_s": round(cells / wall / 1e6, 1),
-            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
-            "p99_latency_s": round(float(np.percentile(lat, 99)), 4)}
+        reqs.append((cfg, seeded_vectors(1, dim, seed + 1 + i)[0]))
+    return reqs
+
+
+def run_serve(ctx, n_req, rates, plain_sample, batch_sched, plain_sched, seed=20261205):
+    """Continuous batching under Poisson arrivals at each rate: requests are
+    submitted at their arrival times while others are resident and retire as
+    they finish; p50/p99 latency is submit -> result.  The plain path serves the
+    same mix one request at a time (its service time is a lower bound of its
+    latency)."""
+    reqs = serve_requests(n_req, seed)
+    for c in {(r[0].nx, r[0].n_exc) for r in reqs}:  # targets are set-up, not timed
+        ctx.target_farfield(t.preset(t.PRESET_CFG5, c[0]).copy(n_exc=c[1]))
+    cells = sum(cell_steps(r[0]) for r in reqs)
+    out = {"workload": f"{n_req} requests, grids {SERVE_GRIDS[0]}^2..{SERVE_GRIDS[-1]}^2 log-spaced, "
+                       f"antennas {SERVE_ANTENNAS}, 8-32-dim vectors, Poisson arrivals (CFG5)",
+           "mean_cell_steps_per_request": cells / n_req, "sweep": []}
+    ctx.set_schedule(**batch_sched)
+    rng = np.random.default_rng(seed + 7)
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        t0 = time.perf_counter()
+        pending, lat, done, i, last = {}, [], 0, 0, t0
+        while done < n_req:
+            now = time.perf_counter() - t0
+            while i < n_req and arrivals[i] <= now:
+                pending[ctx.submit(reqs[i][0], reqs[i][1])] = True
+                i += 1
+            if not pending:
+                time.sleep(max(0.0, min(0.01, arrivals[i] - now)))
+                continue
+            ctx.progress()
+            for tk in list(pending):
+                ok, fit, l = ctx.poll(tk)
+                if ok:
+                    lat.append(l)
+                    del pending[tk]
+                    done += 1
+            if time.perf_counter() - last > 10:
+                log(f"serve rate {rate}: {done}/{n_req} done")
+                last = time.perf_counter()
+        wall = time.perf_counter() - t0
+        out["sweep"].append({"rate_per_s": rate, "wall_s": round(wall, 3), "evals_per_s": round(n_req / wall, 3),
+                             "Mcell_steps_per_s": round(cells / wall / 1e6, 1),
+                             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                             "p99_latency_s": round(float(np.percentile(lat, 99)), 4)})
+    if plain_sample:
+        ctx.set_schedule(**plain_sched)
+        svc, cp = [], 0.0
+        for c, v in reqs[:plain_sample]:
+            t1 = time.perf_counter()
+            ctx.eval(c, v)
+            svc.append(time.perf_counter() - t1)
+            cp += cell_steps(c)
+        out["plain"] = {"requests": plain_sample, "evals_per_s": round(len(svc) / sum(svc), 3),
+                        "Mcell_steps_per_s": round(cp / sum(svc) / 1e6, 1),
+                        "p50_service_s": round(float(np.percentile(svc, 50)), 4),
+                        "p99_service_s": round(float(np.percentile(svc, 99)), 4)}
+    return out
+
+
+def run_search(ctx, method, pop, evals, inflight, sched):
+    """The asynchronous search driver (tomo_search_run) on the 512^2,
+    4-antenna serving configuration: a closed loop that issues a new request
+    as soon as a result returns."""
+    cfg = t.preset(t.PRESET_CFG5, 512).copy(n_exc=4)
+    ctx.target_farfield(cfg)
+    ctx.set_schedule(**sched)
+    lo = np.array([-15.0, -15.0, 2.0, 10.0, 10.0, 8.0, 0.0, 0.0])
+    hi = np.array([15.0, 15.0, 6.0, 30.0, 50.0, 20.0, 1.0, 0.0])
+    best, res = ctx.search(cfg, lo, hi, method, pop=pop, max_evals=evals, inflight=inflight, seed=20261206)
+    res = {k: (round(v, 4) if isinstance(v, float) and k != "best_fitness" else v) for k, v in res.items()}
+    res["method"] = "DE" if method == t.SEARCH_DE else "PSO"
+    res.update({"pop": pop, "inflight": inflight, "workload": "512^2, 4 antennas, 8-dim (CFG5 grid)",
+                "Mcell_steps_per_s": round(res["evals"] * cell_steps(cfg) / res["wall_s"] / 1e6, 1),
+                "best_x": [round(float(x), 3) for x in best]})
+    return res
 
 
 def uneven_requests(n, seed=20261204):
@@ -366,8 +420,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study")
-    ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study (per rate)")
+    ap.add_argument("--rate", type=str, default="2", help="Poisson arrival rates (requests/s), comma separated")
+    ap.add_argument("--serve-plain-sample", type=int, default=8, help="plain-path requests timed on the serving mix")
+    ap.add_argument("--search", type=int, default=0, help="evaluations of the asynchronous DE search driver")
     ap.add_argument("--schedule", type=str, default="",
                     help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult,form")
     ap.add_argument("--batch-schedule", type=str, default="", help="batched-path schedule (same fields)")
@@ -423,14 +479,15 @@ def main():
             pl = run_plain_same_workload(ctx, args.plain_evals, sched, bfits)
             out["batched"]["plain_same_workload"] = pl
             out["batched"]["batched_vs_plain_evals_per_s"] = round(out["batched"]["evals_per_s"] / pl["evals_per_s"], 3)
-    if args.serve:
-        ctx.set_schedule(**bsch)  # the many-simulation schedule for the request loop
     if args.uneven:
         out["uneven"] = run_uneven(ctx, args.uneven, args.uneven_plain_sample, bsch, sched)
     if args.precision:
         out["precision"] = run_precision(ctx, args.precision)
     if args.serve:
-        out["serving"] = run_serve(ctx, args.serve, args.rate)
+        out["serving"] = run_serve(ctx, args.serve, [float(r) for r in args.rate.split(",")], args.serve_plain_sample,
+                                   bsch, sched)
+    if args.search:
+        out["search"] = run_search(ctx, t.SEARCH_DE, 256, args.search, 64, bsch)
     if args.calib_copy_mb:
         hip_copy_calibration(args.calib_copy_mb)
     if not args.no_cpu:

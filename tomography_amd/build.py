@@ -3,7 +3,8 @@
     python -m tomography_amd.build          # library + oracle (+ oracle/_ref when the reference tree exists)
 
 Every HIP source is compiled with hipcc for gfx950 only, with packed-FP32
-instruction selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+instruction selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops)
+and multiply-add contraction limited to single expressions (-ffp-contract=on).
 """
 import os
 import subprocess
@@ -17,7 +18,11 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtomo.so")
 ARCH = os.environ.get("TOMO_OFFLOAD_ARCH", "gfx950")
 
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# -ffp-contract=on: multiply-adds are fused within one expression only, so a
+# stage computes bit-identical values whichever specialised variant of the
+# surrounding code (full / partial time block, with / without the source
+# cell) it is compiled into.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 SOURCES = ["kernels.hip", "setup.cpp", "runtime.cpp", "search.cpp", "fdtd_gpu_shim.cpp"]

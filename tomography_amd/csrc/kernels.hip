@@ -21,6 +21,8 @@
 // that overlapping ghost reads never see a neighbour's update.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace tomo {
@@ -561,12 +563,15 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
     }
     auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
     auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c); };
-    auto step = [&](int r, Cell in) {
+    // FAST: every stage active (nst == T) and no source cell in reach of the tile,
+    // so the stage loop carries no per-stage branch.
+    auto stepg = [&](auto fast, int r, Cell in) {
+        constexpr bool FAST = decltype(fast)::value;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             Cell c = cur[t];
-            if (t < nst) {
+            if (FAST || t < nst) {
                 // wave-uniform row coefficients
                 const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
                 const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
@@ -591,7 +596,7 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
                 const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                 float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
-                if (wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
+                if (!FAST && wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
                 const bool eok = ex_ok && ey_ok;
                 c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
                 c.ez = eok ? en : 0.f;
@@ -604,7 +609,11 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         }
         return in;
     };
-    row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step);
+    auto step_fast = [&](int r, Cell in) { return stepg(std::true_type{}, r, in); };
+    auto step_gen = [&](int r, Cell in) { return stepg(std::false_type{}, r, in); };
+    const bool src_near = wf.src && S.sy >= y0 - 2 * T && S.sy < y1 + T;
+    if (nst == T && !src_near) row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step_fast);
+    else row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step_gen);
 }
 
 template <int T>
@@ -731,12 +740,15 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
             if (yo < nc) ph_out[yo * pitch + x] = c.phy;
             else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = c.phy;
     };
-    auto step = [&](int r, YCell in) {
+    // FAST: every stage active (nst == T) and no source cell in reach of the tile,
+    // so the stage loop carries no per-stage branch.
+    auto stepg = [&](auto fast, int r, YCell in) {
+        constexpr bool FAST = decltype(fast)::value;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             YCell c = cur[t];
-            if (t < nst) {
+            if (FAST || t < nst) {
                 const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
                 const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
                 const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
@@ -753,7 +765,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
                 const float cx = dHx * lcy[iye] + dyf * c.pey;
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                 float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
-                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
                 c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
                 c.ez = eok ? en : 0.f;
                 c.hx = hx;
@@ -778,7 +790,11 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
         }
         return in;
     };
-    row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step);
+    auto step_fast = [&](int r, YCell in) { return stepg(std::true_type{}, r, in); };
+    auto step_gen = [&](int r, YCell in) { return stepg(std::false_type{}, r, in); };
+    const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
+    if (nst == T && !src_near) row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -913,12 +929,15 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
             if (je >= 0) pe_out[yo * w2 + je] = c.pex;
             if (jh >= 0) ph_out[yo * w2 + jh] = c.phx;
     };
-    auto step = [&](int r, XCell in) {
+    // FAST: every stage active (nst == T) and no source cell in reach of the tile,
+    // so the stage loop carries no per-stage branch.
+    auto stepg = [&](auto fast, int r, XCell in) {
+        constexpr bool FAST = decltype(fast)::value;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             XCell c = cur[t];
-            if (t < nst) {
+            if (FAST || t < nst) {
                 const float dEx = lane_next(c.ez) - c.ez;
                 const float dEy = in.ez - c.ez;
                 c.phx = bxh * c.phx + axh * dEx;
@@ -932,7 +951,7 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
                 const float cy = dHy * kxe + dxf * c.pex;
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                 float en = qa.x * c.ez + qa.y * cy - qa.y * dHx - qb.x * c.jp;
-                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
                 c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
                 c.ez = eok ? en : 0.f;
                 c.hx = hx;
@@ -957,7 +976,11 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
         }
         return in;
     };
-    row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step);
+    auto step_fast = [&](int r, XCell in) { return stepg(std::true_type{}, r, in); };
+    auto step_gen = [&](int r, XCell in) { return stepg(std::false_type{}, r, in); };
+    const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
+    if (nst == T && !src_near) row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1050,12 +1073,15 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
         cur[t].mat = 0;
         hxp[t] = 0.f;
     }
-    auto step = [&](int r, ICell in) {
+    // FAST: every stage active (nst == T) and no source cell in reach of the tile,
+    // so the stage loop carries no per-stage branch.
+    auto stepg = [&](auto fast, int r, ICell in) {
+        constexpr bool FAST = decltype(fast)::value;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             ICell c = cur[t];
-            if (t < nst) {
+            if (FAST || t < nst) {
                 // H half-step
                 const float dEx = lane_next(c.ez) - c.ez;
                 const float dEy = in.ez - c.ez;
@@ -1067,7 +1093,7 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
                 hxp[t] = hx;
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                 float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
-                if (wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
                 c.jp = qb.y * c.jp + qb.z * (en - c.ez);
                 c.ez = en;
                 c.hx = hx;
@@ -1078,7 +1104,11 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
         }
         return in;
     };
-    row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step);
+    auto step_fast = [&](int r, ICell in) { return stepg(std::true_type{}, r, in); };
+    auto step_gen = [&](int r, ICell in) { return stepg(std::false_type{}, r, in); };
+    const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
+    if (nst == T && !src_near) row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1255,23 +1285,33 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
 //     C[(s, c), a] = sum_p X[(s, c), p] W[p, a],   W[p, a] = e^{jk(x_p cos phi_a + y_p sin phi_a)},
 // with four rows per simulation: c = 0 carries -del_p J_p (-> N), c = 1
 // del_p M_p on y faces (-> L_y), c = 2 del_p M_p on x faces (-> L_x), c = 3
-// zero.  Each wave owns 32 angles x kProjTiles*32 rows and walks the contour
-// two points at a time with v_mfma_f32_32x32x2f32, four products per complex
-// step (Re = XrWr - XiWi, Im = XrWi + XiWr).  Operand layouts of the 32x32x2
-// f32 product: A[i][k] from lane i + 32k, B[k][j] from lane j + 32k, and
-// accumulator register r of lane l holds C[8(r/4) + 4(l/32) + r%4][l%32], so a
-// lane ends up holding all four rows of simulations 2q + l/32 (q = r/4) at
-// angle l%32 and forms D and the phase without leaving its registers.
+// zero.  A workgroup of four waves owns 32 simulations (8 per wave, one
+// 32-row tile each) x kProjAngles angles.  The contour is walked in chunks of
+// kProjChunk points: the workgroup evaluates the chunk's phase tile W once
+// into LDS, then every wave runs v_mfma_f32_32x32x2f32 over the chunk, two
+// points at a time, four products per complex step (Re = XrWr - XiWi,
+// Im = XrWi + XiWr) for each 32-angle tile, reusing its current values X
+// across the angle tiles.  Operand layouts of the 32x32x2 f32 product:
+// A[i][k] from lane i + 32k, B[k][j] from lane j + 32k, and accumulator
+// register r of lane l holds C[8(r/4) + 4(l/32) + r%4][l%32], so a lane ends
+// up holding all four rows of simulations 2q + l/32 (q = r/4) at angle l%32
+// and forms D and the phase without leaving its registers.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-constexpr int kProjTiles = 4;  // 32-row tiles (8 simulations each) per wave
+constexpr int kProjAngleTiles = 4;                 // 32-angle tiles per workgroup
+constexpr int kProjAngles = 32 * kProjAngleTiles;
+constexpr int kProjChunk = 16;                     // contour points per LDS phase tile
+constexpr int kProjWaves = 4;                      // waves (8 simulations each) per workgroup
 
-__global__ __launch_bounds__(64) void k_project_mfma(const int* __restrict__ sim_idx, int n_sims,
-                                                     const DevSim* __restrict__ sims,
-                                                     const float2* __restrict__ kinv, int A, float eta0,
-                                                     float* __restrict__ D, float* __restrict__ Pph) {
-    const int lane = threadIdx.x;
-    const int a0 = blockIdx.x * 32, s0 = blockIdx.y * (8 * kProjTiles), f = blockIdx.z;
-    const DevSim* G = sims + sim_idx[0];  // geometry shared by the group
+__global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __restrict__ sim_idx, int n_sims,
+                                                                  const DevSim* __restrict__ sims,
+                                                                  const float2* __restrict__ kinv, int A,
+                                                                  float eta0, float* __restrict__ D,
+                                                                  float* __restrict__ Pph) {
+    __shared__ float2 wt[kProjChunk][kProjAngles];  // phase tile {cos, sin}
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int a0 = blockIdx.x * kProjAngles, f = blockIdx.z;
+    const int s0 = (blockIdx.y * kProjWaves + wv) * 8;  // this wave's first simulation
+    const DevSim* G = sims + sim_idx[0];                 // geometry shared by the group
     const int P = G->P, L1 = G->L1, L2 = G->L2, L3 = G->L3, F = G->nfreq;
     const bool fp64 = (G->flags & TOMO_FLAG_DFT_FP64) != 0;
     const float2 ki = kinv[f];
@@ -1279,76 +1319,74 @@ __global__ __launch_bounds__(64) void k_project_mfma(const int* __restrict__ sim
     const float dxf = G->dxf, dyf = G->dyf;
     const float* __restrict__ px = G->pos;
     const float* __restrict__ py = G->pos + P;
-    // B operand lane: angle a0 + lane%32, point p0 + lane/32
-    const int ab = a0 + (lane & 31);
-    float cph = 0.f, sph = 0.f;
-    sincosf(6.283185307179586f * (float)ab / (float)A, &sph, &cph);
-    const bool a_ok = ab < A;
-    // A operand lane: row lane%32 of each tile -> simulation (lane%32)/4, row kind (lane%32)%4
-    const int kh = lane >> 5;
-    const int rowk = lane & 3;
-    const void* accJ[kProjTiles];
-    const void* accM[kProjTiles];
-#pragma unroll
-    for (int m = 0; m < kProjTiles; ++m) {
-        const int s = s0 + 8 * m + ((lane & 31) >> 2);
-        accJ[m] = accM[m] = nullptr;
-        if (s < n_sims) {
-            const DevSim* S = sims + sim_idx[s];
-            accJ[m] = S->accJ;
-            accM[m] = S->accM;
-        }
+    // phase-tile filler: thread i covers angle a0 + i % kProjAngles for points (i / kProjAngles) + 2j
+    const int fa = threadIdx.x % kProjAngles, fp = threadIdx.x / kProjAngles;
+    float fc = 0.f, fs = 0.f;
+    sincosf(6.283185307179586f * (float)(a0 + fa) / (float)A, &fs, &fc);
+    // A operand lane: simulation s0 + (lane%32)/4, row kind (lane%32)%4, point p0 + lane/32
+    const int kh = lane >> 5, rowk = lane & 3;
+    const int sl = s0 + ((lane & 31) >> 2);
+    const void* accX = nullptr;  // J for row kind 0, M for kinds 1 and 2
+    if (sl < n_sims && rowk != 3) {
+        const DevSim* S = sims + sim_idx[sl];
+        accX = rowk == 0 ? S->accJ : S->accM;
     }
-    f32x16 cre[kProjTiles], cim[kProjTiles];
+    f32x16 cre[kProjAngleTiles], cim[kProjAngleTiles];
 #pragma unroll
-    for (int m = 0; m < kProjTiles; ++m)
+    for (int m = 0; m < kProjAngleTiles; ++m)
 #pragma unroll
         for (int r = 0; r < 16; ++r) cre[m][r] = cim[m][r] = 0.f;
-    for (int p0 = 0; p0 < P; p0 += 2) {
-        const int p = p0 + kh;
-        float wr = 0.f, wi = 0.f, del = 0.f;
-        bool ysf = false;
-        if (p < P) {
-            if (a_ok) {
-                float sn, cs;
-                sincosf(k * (px[p] * cph + py[p] * sph), &sn, &cs);
-                wr = cs;
-                wi = sn;
-            }
-            ysf = p < L1 || (p >= L2 && p < L3);
-            del = ysf ? dxf : dyf;
-        }
+    for (int c0 = 0; c0 < P; c0 += kProjChunk) {
+        __syncthreads();
 #pragma unroll
-        for (int m = 0; m < kProjTiles; ++m) {
-            float xr = 0.f, xi = 0.f;
-            if (p < P && rowk != 3 && accJ[m]) {
-                const size_t o = (size_t)f * P + p;
-                float vr, vi;
-                if (rowk == 0) {
-                    if (fp64) { const double2 v = ((const double2*)accJ[m])[o]; vr = (float)v.x; vi = (float)v.y; }
-                    else { const float4 v = ((const float4*)accJ[m])[o]; vr = v.x; vi = v.y; }
-                    xr = -del * vr;
-                    xi = -del * vi;
-                } else if ((rowk == 1) == ysf) {
-                    if (fp64) { const double2 v = ((const double2*)accM[m])[o]; vr = (float)v.x; vi = (float)v.y; }
-                    else { const float4 v = ((const float4*)accM[m])[o]; vr = v.x; vi = v.y; }
-                    xr = del * vr;
-                    xi = del * vi;
+        for (int j = 0; j < kProjChunk / 2; ++j) {
+            const int pp = fp + 2 * j, p = c0 + pp;
+            float sn = 0.f, cs = 0.f;
+            if (p < P && a0 + fa < A) sincosf(k * (px[p] * fc + py[p] * fs), &sn, &cs);
+            wt[pp][fa] = make_float2(cs, sn);
+        }
+        // this lane's current values for the chunk (two points per step)
+        float xr[kProjChunk / 2], xi[kProjChunk / 2];
+#pragma unroll
+        for (int j = 0; j < kProjChunk / 2; ++j) {
+            const int p = c0 + 2 * j + kh;
+            xr[j] = xi[j] = 0.f;
+            if (p < P && accX) {
+                const bool ysf = p < L1 || (p >= L2 && p < L3);
+                if (rowk == 0 || (rowk == 1) == ysf) {
+                    const float del = (ysf ? dxf : dyf) * (rowk == 0 ? -1.f : 1.f);
+                    const size_t o = (size_t)f * P + p;
+                    float vr, vi;
+                    if (fp64) { const double2 v = ((const double2*)accX)[o]; vr = (float)v.x; vi = (float)v.y; }
+                    else { const float4 v = ((const float4*)accX)[o]; vr = v.x; vi = v.y; }
+                    xr[j] = del * vr;
+                    xi[j] = del * vi;
                 }
             }
-            cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr, wr, cre[m], 0, 0, 0);
-            cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(-xi, wi, cre[m], 0, 0, 0);
-            cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr, wi, cim[m], 0, 0, 0);
-            cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi, wr, cim[m], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kProjChunk / 2; ++j) {
+#pragma unroll
+            for (int m = 0; m < kProjAngleTiles; ++m) {
+                const float2 w = wt[2 * j + kh][32 * m + (lane & 31)];
+                cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.x, cre[m], 0, 0, 0);
+                cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(-xi[j], w.y, cre[m], 0, 0, 0);
+                cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.y, cim[m], 0, 0, 0);
+                cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi[j], w.x, cim[m], 0, 0, 0);
+            }
         }
     }
-    if (!a_ok) return;
     const float inv = ki.y;
 #pragma unroll
-    for (int m = 0; m < kProjTiles; ++m) {
+    for (int m = 0; m < kProjAngleTiles; ++m) {
+        const int ab = a0 + 32 * m + (lane & 31);
+        if (ab >= A) continue;
+        float cph, sph;
+        sincosf(6.283185307179586f * (float)ab / (float)A, &sph, &cph);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int s = s0 + 8 * m + 2 * q + kh;
+            const int s = s0 + 2 * q + kh;
             if (s >= n_sims) continue;
             const float Nr = cre[m][4 * q], Ni = cim[m][4 * q];
             const float Lyr = cre[m][4 * q + 1], Lyi = cim[m][4 * q + 1];
@@ -1538,7 +1576,8 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
 
 hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const DevSim* sims, const float2* kinv,
                                int n_angles, float eta0, float* D, float* Pph, hipStream_t st) {
-    dim3 g((n_angles + 31) / 32, (n_sims + 8 * kProjTiles - 1) / (8 * kProjTiles), n_freq), b(64);
+    dim3 g((n_angles + kProjAngles - 1) / kProjAngles, (n_sims + 8 * kProjWaves - 1) / (8 * kProjWaves), n_freq),
+        b(64 * kProjWaves);
     hipLaunchKernelGGL(k_project_mfma, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
     return hipGetLastError();
 }
