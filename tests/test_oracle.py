@@ -211,3 +211,85 @@ def test_planted_mistake_is_caught(oracle):
     g = np.load(os.path.join(GOLD, "ref_cpu_cpml.npz"))["tables"]
     s = oracle.sim(ref_cpu_full().copy(cpml_order=3), ref_cpu_params(), exc=0)
     assert not np.array_equal(s.cpml().astype(np.float32), g)
+
+
+def _debye_coefficients(einf, deps, tau, sigma, dt, dx):
+    """First principles: the trapezoidal-in-time discretisation at one node of
+        eps0 einf dE/dt + sigma E + Jp = curl H,   tau dJp/dt + Jp = eps0 deps dE/dt,
+    solved as a 2x2 linear system for (E^{n+1}, Jp^{n+1}) from (E^n, Jp^n,
+    curl).  Returns the E coefficients on E^n, on (dx curl), on Jp^n (negated)
+    and the Jp recursion Jp^{n+1} = Kp Jp^n + g (E^{n+1} - E^n)."""
+    a = EPS0 * einf / dt
+    if tau > 0:
+        q = tau / dt
+        kp, g = (q - 0.5) / (q + 0.5), EPS0 * deps / dt / (q + 0.5)
+    else:
+        kp, g = 0.0, 0.0
+    lhs = np.array([[a + sigma / 2, 0.5], [-g, 1.0]])
+    col = {}
+    for name, e0, j0, cu in (("E", 1.0, 0.0, 0.0), ("J", 0.0, 1.0, 0.0), ("C", 0.0, 0.0, 1.0)):
+        rhs = np.array([(a - sigma / 2) * e0 - 0.5 * j0 + cu, kp * j0 - g * e0])
+        col[name] = np.linalg.solve(lhs, rhs)
+    # curl = (dHy - dHx) / dx, so the coefficient of (dHy - dHx) is col["C"][0] / dx
+    return col["E"][0], col["C"][0], -col["J"][0], kp, g
+
+
+def test_physical_debye_coefficients_follow_from_the_discretised_pdes(oracle):
+    """The physical-mode E / Debye-current coefficients of every tissue equal
+    the solution of the trapezoidal discretisation of Ampere's law with a
+    single-pole Debye current (an independent 2x2 linear solve in FP64, not the
+    closed-form expressions of the restatement), to FP32 rounding."""
+    cfg = phys_small()
+    p = np.array([3.0, -2.0, 4.0, 25.0, 40.0, 10.0, 0.5, 0.0])  # tumour einf, deps, tau (ps), sigma
+    s = oracle.sim(cfg, p, exc=0)
+    mat, oc = s.materials()
+    tissues = {1: (36.0, 0.0, 0.0, 4.0), 2: (3.1, 1.6, 13e-12, 0.0), 3: (12.8, 24.6, 13e-12, 0.0),
+               4: (25.0, 40.0, 10e-12, 0.5)}
+    seen = 0
+    for m, (ei, de, tau, sg) in tissues.items():
+        sel = mat == m
+        if not sel.any():
+            continue
+        seen += 1
+        ceze, cezh, cezjp, kp, g = _debye_coefficients(ei, de, tau, sg, cfg.dt, cfg.dx)
+        got = [oc[0][sel][0], oc[1][sel][0] * cfg.dx, oc[5][sel][0], oc[8][sel][0], oc[7][sel][0]]
+        want = [ceze, cezh, cezjp, kp, g]
+        for gv, wv, name in zip(got, want, ("Ceze", "Cezh*dx", "Cezjp", "Kp", "Bp/dt")):
+            assert abs(gv - wv) <= 2e-7 * abs(wv) + 1e-30, (m, name, gv, wv)
+    assert seen == 4
+
+
+def _cpml_reflection(oracle, **mod):
+    """Relative difference of a probe trace 5 cells in front of the right CPML
+    of a 128^2 grid against the same source/probe offsets in a 512^2 grid whose
+    boundary is too far away for a reflection to return within the run."""
+    p = PHYS_PARAMS.copy()
+    p[2] = 0.0
+    steps = 260
+    traces, offset = [], None
+    for n in (128, 512):
+        cfg = phys_small(steps=steps, n=n, n_exc=1).copy(breast_radius=0.0002, skin_thickness=0.0001,
+                                                        src_radius=0.010, **mod)
+        sx, sy = oracle.source_cell(cfg, 0)
+        if offset is None:
+            offset = (n - cfg.ncpml - 5 - 1) - sx  # probe - source, in cells, fixed by the small grid
+        s = oracle.sim(cfg, p, exc=0, fp64=True)
+        tr = []
+        for _ in range(steps):
+            s.step(1)
+            tr.append(s.probe(sx + offset, sy))
+        traces.append(np.array(tr))
+    small, big = traces
+    assert np.abs(big).max() > 0 and offset > 10
+    return np.abs(small - big).max() / np.abs(big).max()
+
+
+def test_physical_cpml_reflection_is_small(oracle):
+    """Closed form by construction: the only difference between a small grid
+    and a large one, at a probe in front of the small grid's CPML and before
+    any reflection can return in the large grid, is the CPML's reflection.  The
+    graded profile (m = 3, R0 = 1e-6) measures 3.5e-6 of the peak field here;
+    the bar is 1e-4.  Planted mistake: the same profile without its loss
+    (sigma_max = 0, a lossless coordinate stretch) must reflect visibly."""
+    assert _cpml_reflection(oracle) < 1e-4
+    assert _cpml_reflection(oracle, sigma_max=0.0) > 1e-2
