@@ -33,6 +33,23 @@
 // keeps the same update structure (Yee leapfrog, Roden-Gedney CPML, the same
 // two-level Debye polarisation current, the same running DFT and projection)
 // with the reference's non-physical quirks removed.
+//
+// How each part is pinned (tests/test_oracle.py):
+//   CPU-path fields, CPML tables, contour, materials, running DFT:
+//       bit-exact against fixtures from the compiled FDTD_CPU.cxx;
+//   GPU-path E update: the delta_eps = 0 special case reduces it to the CPU
+//       fixture bit for bit; a vacuum phantom gives D == 0;
+//   calc_incident_power: numerical transform of the Gaussian pulse;
+//   physical Debye coefficients: an independent 2x2 solve of the discretised
+//       PDEs; physical CPML: reflection against a larger grid; physical running
+//       DFT, closed contour and projection: point-source isotropy and a direct
+//       transform of the samples.
+//   Parity unpinned (FDTD_GPU.cu cannot be compiled here and its known-answer
+//   candidate, the commented vector of FDTD_GPU.cu:659-692 against
+//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the GPU
+//   path's Debye current with delta_eps > 0, the centred fwf origin, the phase
+//   output P of N2FPostProcess and the GPU fitness normalisation are restated
+//   from the source only.
 
 #include <cmath>
 #include <complex>
