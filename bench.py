@@ -115,11 +115,12 @@ def kernel_ms(st, per=1.0):
     return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
 
 
-SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form")
+SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form", "edge_rows")
 # Default schedules (picked by Context.autotune on MI355X; fixed so that the
 # profiled command and the timed one run the same launches).
-PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=8, edge_waves=4, inner_waves=2, chunk=32, grid_mult=16, form=2)
-BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=16, form=1)
+PLAIN_SCHEDULE = dict(tsteps=8, seg_rows=16, edge_waves=4, inner_waves=4, chunk=32, grid_mult=16, form=2, edge_rows=2)
+BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=16, form=1,
+                      edge_rows=8)
 
 
 def apply_schedule(ctx, cfg, n_sims, sched, tune):
@@ -425,7 +426,7 @@ def main():
     ap.add_argument("--serve-plain-sample", type=int, default=8, help="plain-path requests timed on the serving mix")
     ap.add_argument("--search", type=int, default=0, help="evaluations of the asynchronous DE search driver")
     ap.add_argument("--schedule", type=str, default="",
-                    help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult,form")
+                    help="plain-path schedule tsteps,seg_rows,edge_waves,inner_waves,chunk,grid_mult,form,edge_rows")
     ap.add_argument("--batch-schedule", type=str, default="", help="batched-path schedule (same fields)")
     ap.add_argument("--autotune", action="store_true", help="pick both schedules with the autotuner instead")
     ap.add_argument("--plain-evals", type=int, default=2,

@@ -133,18 +133,20 @@ const char *tomo_last_error(void);
  * steps (a multiple of tsteps, <= 64); grid_mult: persistent workgroups per CU;
  * form: how a physical-mode time block is launched, TOMO_FORM_STREAMS (one
  * kernel per tile class on four streams, joined per block) or
- * TOMO_FORM_FUSED (one launch of a kernel that runs every class). */
+ * TOMO_FORM_FUSED (one launch of a kernel that runs every class);
+ * edge_rows: output rows per corner and row-edge tile (the latency-critical
+ * boundary tiles; at most seg_rows). */
 #define TOMO_FORM_STREAMS 1
 #define TOMO_FORM_FUSED 2
 typedef struct tomo_schedule {
-    int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form;
+    int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows;
 } tomo_schedule;
 int tomo_set_schedule(tomo_ctx *ctx, const tomo_schedule *s);
 int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
 
 /* Autotuner: time candidate schedules on n_sims resident simulations of
- * `cfg` (coordinate descent over form, tsteps, seg_rows, edge/inner waves
- * and grid_mult; the chunk is kept), install the fastest and return it with its
+ * `cfg` (coordinate descent over form, tsteps, seg_rows, edge_rows,
+ * edge/inner waves and grid_mult; the chunk is kept), install the fastest and return it with its
  * measured update rate in cell-steps per second. */
 int tomo_autotune(tomo_ctx *ctx, const tomo_sim_config *cfg, int n_sims, tomo_schedule *best,
                   double *cell_steps_per_s);

@@ -101,6 +101,7 @@ struct tomo_ctx {
     int T = 4, S = 32, W = 4, K = 32;  // schedule (W: waves per edge workgroup)
     int grid_mult = 4;                 // persistent workgroups per CU
     int form = TOMO_FORM_STREAMS;      // launch form of physical-mode blocks
+    int ER = 8;                        // output rows per corner / row-edge tile
     bool use_graph = true;
     // DevSim table
     DevSim* d_sims = nullptr;
@@ -116,7 +117,7 @@ struct tomo_ctx {
     hipEvent_t staged = nullptr;
     std::vector<uint64_t> tile_key;  // serials of the simulations the uploaded tile list was built for
     uint64_t next_serial = 1;
-    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0;
+    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0, tile_key_ER = 0;
     int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
     int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
     int n_corner = 0;
@@ -372,7 +373,7 @@ int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); 
 // full-width bands of the clean interior go to the interior kernel (physical
 // mode only).  Interior bands start exactly where the clean zone starts, so
 // the edge frame is one narrow band wide on each side.
-void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<TileItem>& edge,
+void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, int ER, std::vector<TileItem>& edge,
                  std::vector<TileItem>& rowedge, std::vector<TileItem>& coledge, std::vector<TileItem>& inner) {
     const DevSim& d = sl->d;
     for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
@@ -423,13 +424,15 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, std::vector<Ti
     // top and bottom rows: corner bands to the general kernel, clean-column
     // bands (full width, lanes inside [xbot, xtop]) to the row-edge kernel
     auto edge_rows_split = [&](int y0, int y1) {
-        const int sc = std::min(S, 8);  // corner tiles: short row chunks (their code path is the slowest)
+        // corner and row-edge tiles: short row chunks (their code path is the
+        // slowest, and a tile's latency grows with its rows)
+        const int sc = std::min(S, ER);
         for (int a = y0; a < y1; a += sc) {
             const int a1 = std::min(a + sc, y1);
             add(0, edge, 0, ixl, a, a1, W);
             add(0, edge, ixr, nx + 1, a, a1, W);
         }
-        for (int a = y0; a < y1; a += S) add(1, rowedge, ixl, ixr, a, std::min(a + S, y1), Wi);
+        for (int a = y0; a < y1; a += sc) add(1, rowedge, ixl, ixr, a, std::min(a + sc, y1), Wi);
     };
     edge_rows_split(0, ia);
     for (int a = ia; a < ib; a += S) {
@@ -510,9 +513,10 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             for (Slot* s : ms) key.push_back(s->serial);
             std::vector<TileItem> te, tr, tc, tn;
             const bool rebuild = key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S ||
-                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi;
+                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi || c->tile_key_ER != c->ER;
             if (rebuild) {
-                for (size_t i = 0; i < ms.size(); ++i) build_tiles(ms[i], (int)i, T, S, W, c->Wi, te, tr, tc, tn);
+                for (size_t i = 0; i < ms.size(); ++i)
+                    build_tiles(ms[i], (int)i, T, S, W, c->Wi, c->ER, te, tr, tc, tn);
             }
             const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : c->ntiles;
             int rc = ensure_staging(c, (int)ms.size(), ntiles);
@@ -541,6 +545,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                                       c->stream));
                 c->tile_key = key;
                 c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W; c->tile_key_Wi = c->Wi;
+                c->tile_key_ER = c->ER;
                 c->ntiles = ntiles;
                 c->n_edge = (int)(te.size() + tr.size() + tc.size());
                 c->n_corner = (int)te.size();
@@ -1066,11 +1071,12 @@ static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
     const int W = s.edge_waves ? s.edge_waves : c->W, Wi = s.inner_waves ? s.inner_waves : c->Wi;
     const int K = s.chunk ? s.chunk : c->K, G = s.grid_mult ? s.grid_mult : c->grid_mult;
     const int F = s.form ? s.form : c->form;
+    const int ER = s.edge_rows ? s.edge_rows : c->ER;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
     if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64 ||
-        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED))
+        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED) || ER < 1 || ER > 4096)
         return fail(TOMO_E_INVALID, "bad schedule");
-    c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F;
+    c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F; c->ER = ER;
     return TOMO_OK;
 }
 
@@ -1086,7 +1092,7 @@ int tomo_set_schedule(tomo_ctx* c, const tomo_schedule* s) {
 int tomo_get_schedule(tomo_ctx* c, tomo_schedule* s) {
     if (!c || !s) return fail(TOMO_E_INVALID, "null argument");
     s->tsteps = c->T; s->seg_rows = c->S; s->edge_waves = c->W; s->inner_waves = c->Wi;
-    s->chunk = c->K; s->grid_mult = c->grid_mult; s->form = c->form;
+    s->chunk = c->K; s->grid_mult = c->grid_mult; s->form = c->form; s->edge_rows = c->ER;
     return TOMO_OK;
 }
 
@@ -1136,7 +1142,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     }
     const bool prof = c->prof;
     c->prof = false;
-    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form};
+    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form, c->ER};
     double cells = 0;
     for (Slot* s : sl) cells += (double)s->d.nx * s->d.ny * c->K;
     double best_t = 1e30;
@@ -1151,12 +1157,14 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     };
     if (!rc) rc = trial(cur);
     const int Ts[] = {1, 2, 4, 8}, Ss[] = {8, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
+    const int ERs[] = {2, 4, 8, 16};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
             for (int v : Fs) { tomo_schedule x = cur; x.form = v; if (!rc) rc = trial(x); }
         for (int v : Ts) { tomo_schedule x = cur; x.tsteps = v; if (!rc && c->K % v == 0) rc = trial(x); }
         for (int v : Ss) { tomo_schedule x = cur; x.seg_rows = v; if (!rc) rc = trial(x); }
+        for (int v : ERs) { tomo_schedule x = cur; x.edge_rows = v; if (!rc) rc = trial(x); }
         for (int v : Ws) { tomo_schedule x = cur; x.edge_waves = v; if (!rc) rc = trial(x); }
         for (int v : Ws) { tomo_schedule x = cur; x.inner_waves = v; if (!rc) rc = trial(x); }
         for (int v : Gs) { tomo_schedule x = cur; x.grid_mult = v; if (!rc) rc = trial(x); }
