@@ -121,6 +121,7 @@ SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_
 PLAIN_SCHEDULE = dict(tsteps=8, seg_rows=16, edge_waves=4, inner_waves=4, chunk=32, grid_mult=16, form=2, edge_rows=2)
 BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=16, form=1,
                       edge_rows=8)
+REF_SCHEDULE = dict(tsteps=2, seg_rows=8, edge_waves=2, inner_waves=2, chunk=32, grid_mult=8, form=2, edge_rows=4)
 
 
 def apply_schedule(ctx, cfg, n_sims, sched, tune):
@@ -388,6 +389,52 @@ def cpu_baseline(steps=2000):
             "host": platform_host()}
 
 
+def run_ref_compat(ctx, sched, n_eval=2, ref_steps=250):
+    """The drop-in path: FDTD_GPU() and FDTD_CPU() semantics (401^2, 4
+    excitations x 2000 steps, 10 freq x 100 angles) evaluated through the C ABI,
+    beside the reference's own FDTD_CPU.cxx compiled from its sources
+    (oracle/_ref, when present) on a bounded sample: one excitation, the first
+    `ref_steps` steps, extrapolated to a whole evaluation."""
+    ctx.set_schedule(**sched)
+    out = {"schedule": dict(zip(SCHED_KEYS, ctx.schedule()))}
+    rng = np.random.default_rng(20261207)
+    for name, pid in (("FDTD_GPU", t.PRESET_REF_GPU), ("FDTD_CPU", t.PRESET_REF_CPU)):
+        cfg = t.preset(pid)
+        n2 = cfg.numpatches ** 2
+        if cfg.flags & t.FLAG_PARAMS_EPS_SIGMA:
+            vecs = [np.concatenate([rng.uniform(1, 7, n2), rng.uniform(0, 0.3, n2)]) for _ in range(n_eval + 1)]
+        else:
+            vecs = [np.concatenate([rng.uniform(1, 7, n2), rng.uniform(1.5, 41, n2), rng.uniform(0, 0.3, n2)])
+                    for _ in range(n_eval + 1)]
+        target = np.zeros(cfg.farfield_len(), np.float32)
+        ctx.eval(cfg, vecs[0], target=target)  # warm-up
+        t0 = time.perf_counter()
+        for v in vecs[1:]:
+            ctx.eval(cfg, v, target=target)
+        dt = (time.perf_counter() - t0) / n_eval
+        out[name] = {"s_per_eval": round(dt, 4), "evals_per_s": round(1 / dt, 3),
+                     "Mcell_steps_per_s": round(cell_steps(cfg) / dt / 1e6, 1)}
+    try:
+        from oracle.binding import Reference
+        if Reference.available():
+            ref = Reference()
+            cfg = t.preset(t.PRESET_REF_CPU)
+            v = np.concatenate([rng.uniform(1, 7, 81), rng.uniform(0, 0.3, 81)])
+            t0 = time.perf_counter()
+            ref.run(v, 0, ref_steps, want_d=False)
+            dt = time.perf_counter() - t0
+            per_eval = dt * (cfg.n_exc * cfg.steps) / ref_steps
+            out["reference_FDTD_CPU"] = {
+                "kind": "reference", "cores": 1, "s_per_eval": round(per_eval, 2),
+                "Mcell_steps_per_s": round(cfg.nx * cfg.ny * ref_steps / dt / 1e6, 2),
+                "sample": f"FDTD_CPU.cxx compiled from the reference sources (oracle/_ref), excitation 0, "
+                          f"steps 0-{ref_steps - 1} of 2000, 1 thread; {dt:.1f} s, extrapolated to 4 x 2000 steps"}
+            out["FDTD_CPU_gpu_vs_reference_cpu"] = round(per_eval / out["FDTD_CPU"]["s_per_eval"], 1)
+    except (ImportError, OSError):
+        pass
+    return out
+
+
 def platform_host():
     model = ""
     try:
@@ -421,6 +468,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref", action="store_true", help="skip the reference-compatible (drop-in) leg")
     ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study (per rate)")
     ap.add_argument("--rate", type=str, default="2", help="Poisson arrival rates (requests/s), comma separated")
     ap.add_argument("--serve-plain-sample", type=int, default=8, help="plain-path requests timed on the serving mix")
@@ -491,6 +539,8 @@ def main():
         out["search"] = run_search(ctx, t.SEARCH_DE, 256, args.search, 64, bsch)
     if args.calib_copy_mb:
         hip_copy_calibration(args.calib_copy_mb)
+    if not args.no_ref:
+        out["ref_compat"] = run_ref_compat(ctx, REF_SCHEDULE)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["ratio_gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

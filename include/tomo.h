@@ -275,7 +275,9 @@ int tomo_sim_step(tomo_sim *sim, int nsteps);
  * reference indexing x + y*(nx+1) and x + y*nx (FDTD_common.hxx:65). */
 int tomo_sim_fields(tomo_sim *sim, float *ez, float *hx, float *hy, float *jp);
 /* Material id map, (nx+1)*(ny+1) bytes, and the per-material coefficient
- * table (n_mat x 8 floats: Ceze, Cezh, Cezeic, Cezeip, Cezjp, Kp, Beta_p/dt, 0). */
+ * table (n_mat x 8 floats: Ceze, Cezh, Cezeic, Cezeip, Cezjp, Kp, Beta_p/dt,
+ * dispersive flag); `coef` must hold TOMO_MAX_MATERIALS x 8 floats. */
+#define TOMO_MAX_MATERIALS 128
 int tomo_sim_materials(tomo_sim *sim, uint8_t *mat, float *coef, int *n_mat);
 /* Running-DFT accumulators after folding: J and M as interleaved complex
  * [freq][point] (2*n_freq*P floats each), signs and dt applied as in

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import tomography_amd as t
-from cases import PHYS_PARAMS, phys_small, ref_params, ref_small, rel_l2
+from cases import PHYS_PARAMS, phys_small, ref_cpu_params, ref_cpu_small, ref_params, ref_small, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -294,3 +294,19 @@ def test_search_driver(ctx, method):
     assert np.all(best >= lo) and np.all(best <= hi)
     assert ctx.eval(cfg, best) == res["best_fitness"]
     assert res["p99_latency_s"] >= res["p50_latency_s"] > 0
+
+
+def test_ref_cpu_semantics_fields_vs_oracle(ctx, oracle):
+    """The FDTD_CPU() semantics (9 x 9 patches = 82 materials, absolute
+    incident origin, no Debye, no NaN clamp) on the GPU against the FP32
+    oracle, which reproduces the compiled FDTD_CPU.cxx bit for bit."""
+    cfg = ref_cpu_small(steps=300)
+    p = ref_cpu_params(4)
+    g = ctx.sim(cfg, p, exc=1)
+    g.step(300)
+    mat, coef = g.materials()
+    om, _ = oracle.sim(cfg, p, exc=1).materials()
+    assert np.array_equal(mat, om) and coef.shape[0] == 82
+    o32 = oracle_fields(oracle, cfg, p, 1, 300, False).fields()
+    for name, a, b in zip(("ez", "hx", "hy"), g.fields()[:3], o32[:3]):
+        assert rel_l2(a, b) < FIELD_TOL_F32, (name, rel_l2(a, b))

@@ -32,6 +32,8 @@ PRESET_CFG3 = 4
 PRESET_CFG4 = 5
 PRESET_CFG5 = 6
 
+MAX_MATERIALS = 128
+
 FORM_STREAMS = 1
 FORM_FUSED = 2
 

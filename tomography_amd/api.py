@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from ._abi import (SimConfig, TOMO_OK, TOMO_E_PENDING)
+from ._abi import (MAX_MATERIALS, SimConfig, TOMO_OK, TOMO_E_PENDING)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtomo.so")
@@ -314,7 +314,7 @@ class Sim:
     def materials(self):
         c = self.cfg
         mat = np.zeros((c.ny + 1, c.nx + 1), np.uint8)
-        coef = np.zeros(16 * 8, np.float32)
+        coef = np.zeros(MAX_MATERIALS * 8, np.float32)
         n = _c.c_int()
         _check(lib().tomo_sim_materials(self.h, mat.ctypes.data_as(_c.POINTER(_c.c_uint8)), _p(coef, _fp),
                                         _c.byref(n)))

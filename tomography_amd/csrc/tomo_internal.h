@@ -19,7 +19,7 @@
 
 #include "../../include/tomo.h"
 
-#define TOMO_MAX_MAT 16   // materials per simulation (ref: 1 + 3*3 patches)
+#define TOMO_MAX_MAT TOMO_MAX_MATERIALS  // materials per simulation (ref: 1 + numpatches^2, 82 for the CPU path)
 #define TOMO_MAX_NC 32    // CPML cells
 #define TOMO_MAX_FREQ 32
 #define TOMO_NCOEF 8      // Ceze, Cezh, Cezeic, Cezeip, Cezjp, Kp, BpDt, dispersive(0/1)
