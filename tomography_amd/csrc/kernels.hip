@@ -1136,8 +1136,11 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
 // class in TileItem::cls).  One launch per time block with no cross-stream
 // joins: the form for few simulations, where every block's latency is on the
 // critical path.  Classes are listed slowest first so their tiles start first.
+// Three workgroups of 256 threads per CU (<= 168 VGPRs): the corner body would
+// otherwise hold the whole kernel to two, and a 1024^2 grid's tiles would no
+// longer start in one wave of workgroups.
 template <int T>
-__global__ __launch_bounds__(256) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256, 3) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
                                                 const LaunchItem* __restrict__ items,
                                                 const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
