@@ -119,7 +119,7 @@ SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_
 # Default schedules (picked by Context.autotune on MI355X; fixed so that the
 # profiled command and the timed one run the same launches).
 PLAIN_SCHEDULE = dict(tsteps=8, seg_rows=8, edge_waves=4, inner_waves=4, chunk=64, grid_mult=16, form=2, edge_rows=2)
-BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=32, grid_mult=16, form=1,
+BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=64, grid_mult=16, form=1,
                       edge_rows=8)
 REF_SCHEDULE = dict(tsteps=2, seg_rows=8, edge_waves=2, inner_waves=2, chunk=32, grid_mult=8, form=2, edge_rows=4)
 
