@@ -86,9 +86,10 @@ def test_phys_fields_vs_oracle(ctx, oracle):
 
 @pytest.mark.parametrize("sched", [(1, 32, 4, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 3, 48), (8, 64, 2, 64),
                                    (2, 8, 2, 32, 4, 2, t.FORM_FUSED), (8, 5, 4, 64, 1, 1, t.FORM_FUSED),
-                                   (4, 16, 1, 32, 3, 8, t.FORM_FUSED)])
+                                   (4, 16, 1, 32, 3, 8, t.FORM_FUSED), (8, 8, 4, 64, 4, 16, t.FORM_FUSED, 2),
+                                   (4, 32, 1, 64, 4, 16, t.FORM_STREAMS, 3)])
 def test_schedule_does_not_change_results(oracle, sched):
-    """Fused step count, tile height, waves per workgroup, DFT chunk and the
+    """Fused step count, tile heights, waves per workgroup, DFT chunk and the
     launch form only reorder work: every schedule gives the same fields and
     accumulators."""
     cfg = phys_small(steps=200)
