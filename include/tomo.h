@@ -62,6 +62,7 @@ extern "C" {
 #define TOMO_FLAG_PHASE_FITNESS 0x10  /* add the phase term sketched at FDTD_GPU.cu:1009-1018          */
 #define TOMO_FLAG_DFT_FP64 0x20       /* running-DFT accumulators in FP64 instead of Kahan FP32      */
 #define TOMO_FLAG_REL_FITNESS 0x40    /* relative misfit (D-m)^2/m^2 per term (FDTD_CPU.cxx:844)      */
+#define TOMO_FLAG_FITNESS_PER_EXC 0x80 /* normalise by n_exc only, as fitness() at FDTD_common.cxx:53-61 */
 #define TOMO_FLAG_CPU_REF_VARIANT                                                              \
     (TOMO_FLAG_FWF_ABS_ORIGIN | TOMO_FLAG_CPU_COEFS | TOMO_FLAG_NO_NAN_CLAMP |             \
      TOMO_FLAG_PARAMS_EPS_SIGMA | TOMO_FLAG_REL_FITNESS)
@@ -206,6 +207,18 @@ int tomo_eval(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
 int tomo_eval_batch(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
                     size_t n_vec, size_t dim, const float *target, size_t n_target,
                     double *fitness_out, float *D_out);
+
+/*
+ * The objective entry (the FDTD_common.hxx-level role of FDTD_GPU/FDTD_CPU):
+ * parameter vector + preset -> fitness, shared by every caller.  `measurement`
+ * may be NULL: REF_GPU / REF_CPU then use the packaged measurement files (the
+ * arrays FDTD_GPU.cu:996 and FDTD_CPU.cxx:789 hard-code; see the C++ shims)
+ * and physical presets their seeded target phantom.  `flags` are OR-ed into
+ * the preset's (e.g. TOMO_FLAG_FITNESS_PER_EXC | TOMO_FLAG_REL_FITNESS for the
+ * generic fitness() of FDTD_common.cxx:53-61).
+ */
+int tomo_fitness(tomo_ctx *ctx, int preset, int32_t flags, const double *params, size_t n_params,
+                 const float *measurement, size_t n_measurement, double *fitness_out);
 
 /* Far field of the seeded target phantom of a physical configuration. */
 int tomo_target_farfield(tomo_ctx *ctx, const tomo_sim_config *cfg, float *D_out);

@@ -24,7 +24,8 @@
 //   calculate_JandM           FDTD_GPU.cu:231-280,   FDTD_CPU.cxx:146-197
 //   calc_incident_power       FDTD_common.cxx:48-51
 //   N2FPostProcess            FDTD_GPU.cu:553-634,   FDTD_CPU.cxx:538-614
-//   fitness                   FDTD_GPU.cu:1004-1008, FDTD_CPU.cxx:841-845
+//   fitness                   FDTD_GPU.cu:1004-1008, FDTD_CPU.cxx:841-845,
+//                             FDTD_common.cxx:53-61 (TOMO_FLAG_FITNESS_PER_EXC)
 // Deviation from FDTD_CPU.cxx: the far-field loop at FDTD_CPU.cxx:816 reads an
 // uninitialised freq_index (so D stays zero there); the restatement starts
 // that loop at 0, which is what FDTD_GPU.cu:971 does.
@@ -880,7 +881,9 @@ static bool eval_full(const tomo_sim_config& c, const double* p, size_t np, cons
         for (size_t i = 0; i < L; ++i) tgt[i] = target[i];
     }
     double fit = 0;
-    const double norm = (double)c.n_exc * F;
+    // FDTD_GPU.cu:1008 and FDTD_CPU.cxx:844 divide by n_exc * n_freq; the generic
+    // fitness() of FDTD_common.cxx:53-61 by n_exc alone
+    const double norm = (c.flags & TOMO_FLAG_FITNESS_PER_EXC) ? (double)c.n_exc : (double)c.n_exc * F;
     const bool rel = (c.flags & TOMO_FLAG_REL_FITNESS) != 0;  // FDTD_CPU.cxx:844
     for (size_t i = 0; i < L; ++i)
         fit -= (Dall[i] - tgt[i]) * (Dall[i] - tgt[i]) / (rel ? tgt[i] * tgt[i] * norm : norm);

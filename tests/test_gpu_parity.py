@@ -311,3 +311,18 @@ def test_ref_cpu_semantics_fields_vs_oracle(ctx, oracle):
     o32 = oracle_fields(oracle, cfg, p, 1, 300, False).fields()
     for name, a, b in zip(("ez", "hx", "hy"), g.fields()[:3], o32[:3]):
         assert rel_l2(a, b) < FIELD_TOL_F32, (name, rel_l2(a, b))
+
+
+def test_objective_entry_matches_eval(ctx):
+    """tomo_fitness (preset + vector -> fitness, packaged measurement) equals
+    tomo_eval of the same preset against the same measurement; the generic
+    FDTD_common normalisation is n_freq times the relative FDTD_CPU one."""
+    p = ref_params(6)
+    m = t.measurement("gpu")
+    cfg = t.preset(t.PRESET_REF_GPU)
+    f_obj = ctx.fitness(t.PRESET_REF_GPU, p)
+    f_eval = ctx.eval(cfg, p, target=m[:cfg.farfield_len()])
+    assert f_obj == f_eval and f_obj < 0
+    f_rel = ctx.fitness(t.PRESET_REF_GPU, p, flags=t.FLAG_REL_FITNESS)
+    f_com = ctx.fitness(t.PRESET_REF_GPU, p, flags=t.FLAG_REL_FITNESS | t.FLAG_FITNESS_PER_EXC)
+    assert abs(f_com - f_rel * cfg.n_freq) <= 1e-12 * abs(f_com)

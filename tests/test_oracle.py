@@ -293,3 +293,20 @@ def test_physical_cpml_reflection_is_small(oracle):
     (sigma_max = 0, a lossless coordinate stretch) must reflect visibly."""
     assert _cpml_reflection(oracle) < 1e-4
     assert _cpml_reflection(oracle, sigma_max=0.0) > 1e-2
+
+
+def test_generic_fitness_normalisation_matches_fdtd_common(oracle):
+    """fitness() of FDTD_common.cxx:53-61 is -sum (m - D)^2 / (n_exc m^2): with
+    TOMO_FLAG_REL_FITNESS | TOMO_FLAG_FITNESS_PER_EXC the oracle's fitness is
+    the relative form times n_freq (it drops the per-frequency average)."""
+    cfg = ref_small(steps=120).copy(n_exc=2)
+    p = ref_params(4)
+    rng = np.random.default_rng(1)
+    target = rng.uniform(0.01, 0.05, cfg.farfield_len()).astype(np.float32)
+    rel = cfg.copy(flags=cfg.flags | t.FLAG_REL_FITNESS)
+    com = cfg.copy(flags=cfg.flags | t.FLAG_REL_FITNESS | t.FLAG_FITNESS_PER_EXC)
+    f_rel, D, _ = oracle.eval(rel, p, target=target, fp64=True)
+    f_com, _, _ = oracle.eval(com, p, target=target, fp64=True)
+    direct = -float(np.sum((target.astype(np.float64) - D) ** 2 / (cfg.n_exc * target.astype(np.float64) ** 2)))
+    assert abs(f_com - direct) <= 1e-12 * abs(direct)
+    assert abs(f_com - f_rel * cfg.n_freq) <= 1e-12 * abs(f_com)

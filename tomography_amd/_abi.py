@@ -21,6 +21,7 @@ FLAG_PARAMS_EPS_SIGMA = 0x8
 FLAG_PHASE_FITNESS = 0x10
 FLAG_DFT_FP64 = 0x20
 FLAG_REL_FITNESS = 0x40
+FLAG_FITNESS_PER_EXC = 0x80
 FLAG_CPU_REF_VARIANT = (FLAG_FWF_ABS_ORIGIN | FLAG_CPU_COEFS | FLAG_NO_NAN_CLAMP | FLAG_PARAMS_EPS_SIGMA
                         | FLAG_REL_FITNESS)
 

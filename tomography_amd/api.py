@@ -25,7 +25,7 @@ EXPORTS = [
     "tomo_target_farfield", "tomo_submit", "tomo_poll", "tomo_wait", "tomo_cancel", "tomo_progress",
     "tomo_request_bytes", "tomo_sim_create", "tomo_sim_destroy", "tomo_sim_step", "tomo_sim_fields",
     "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_get_stats",
-    "tomo_search_run",
+    "tomo_search_run", "tomo_fitness",
 ]
 
 
@@ -131,6 +131,7 @@ def lib():
     L.tomo_profile.argtypes = [vp, _c.c_int]
     L.tomo_get_stats.argtypes = [vp, _c.c_void_p]
     L.tomo_search_run.argtypes = [vp, vp, _dp, _dp, vp, _dp, vp]
+    L.tomo_fitness.argtypes = [vp, _c.c_int, _c.c_int32, _dp, _c.c_size_t, _fp, _c.c_size_t, _dp]
     _lib = L
     return L
 
@@ -195,6 +196,15 @@ class Context:
         st = Stats()
         _check(lib().tomo_get_stats(self.h, _c.byref(st)))
         return st.as_dict()
+
+    def fitness(self, preset_id, params, flags=0, measurement=None):
+        """The objective entry: preset + vector -> fitness (packaged measurement when None)."""
+        p = np.ascontiguousarray(params, np.float64)
+        m = None if measurement is None else np.ascontiguousarray(measurement, np.float32)
+        f = _c.c_double()
+        _check(lib().tomo_fitness(self.h, preset_id, flags, _p(p, _dp), p.size, _p(m, _fp),
+                                  0 if m is None else m.size, _c.byref(f)))
+        return f.value
 
     def search(self, cfg, lo, hi, method, pop, max_evals, inflight=0, F=0.5, CR=0.9, seed=1):
         """Asynchronous DE / PSO over the request loop: returns (best_x, result dict)."""

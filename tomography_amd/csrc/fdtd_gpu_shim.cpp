@@ -68,31 +68,51 @@ tomo_ctx* ctx() {
     return g_ctx;
 }
 
-double run(int preset, const std::vector<double>& arguments, const char* env, const char* file, bool with_phase,
-           const char* label) {
-    std::lock_guard<std::mutex> g(g_mu);
-    tomo_sim_config cfg;
-    if (tomo_preset(preset, 0, &cfg) != TOMO_OK) throw std::runtime_error(tomo_last_error());
-    if (arguments.size() < (size_t)cfg.n_params) throw std::out_of_range("vector::_M_range_check");
-    const size_t L = (size_t)tomo_farfield_len(&cfg);
+// Packaged measurement of a reference preset (loaded once).
+const std::vector<float>& packaged_measurement(int preset) {
     static std::vector<float> meas_gpu, meas_cpu;
-    std::vector<float>& m = with_phase ? meas_gpu : meas_cpu;
-    if (m.empty()) m = load_measurement(env, file, with_phase ? 2 * L : L);
+    const bool gpu = preset == TOMO_PRESET_REF_GPU;
+    std::vector<float>& m = gpu ? meas_gpu : meas_cpu;
+    if (m.empty()) {
+        tomo_sim_config cfg;
+        if (tomo_preset(preset, 0, &cfg) != TOMO_OK) throw std::runtime_error(tomo_last_error());
+        const size_t L = (size_t)tomo_farfield_len(&cfg);
+        m = gpu ? load_measurement("TOMO_MEASUREMENT_GPU", "measurement_gpu.f32", 2 * L)
+                : load_measurement("TOMO_MEASUREMENT_CPU", "measurement_cpu.f32", L);
+    }
+    return m;
+}
+
+double run(int preset, const std::vector<double>& arguments, const char* label) {
+    std::lock_guard<std::mutex> g(g_mu);
     double fit = 0;
-    const int rc = tomo_eval(ctx(), &cfg, arguments.data(), arguments.size(), m.data(), m.size(), &fit, nullptr,
-                             nullptr);
+    const int rc = tomo_fitness(ctx(), preset, 0, arguments.data(), arguments.size(), nullptr, 0, &fit);
     if (rc == TOMO_E_SHORT) throw std::out_of_range("vector::_M_range_check");
-    if (rc != TOMO_OK) throw std::runtime_error(std::string("tomo_eval: ") + tomo_last_error());
+    if (rc != TOMO_OK) throw std::runtime_error(std::string("tomo_fitness: ") + tomo_last_error());
     std::cout << label << fit << std::endl;
     return fit;
 }
 
 }  // namespace
 
-double FDTD_GPU(const std::vector<double>& arguments) {
-    return run(TOMO_PRESET_REF_GPU, arguments, "TOMO_MEASUREMENT_GPU", "measurement_gpu.f32", true, "fitness is: ");
+extern "C" int tomo_fitness(tomo_ctx* c, int preset, int32_t flags, const double* params, size_t n_params,
+                            const float* measurement, size_t n_measurement, double* fitness_out) {
+    tomo_sim_config cfg;
+    int rc = tomo_preset(preset, 0, &cfg);
+    if (rc) return rc;
+    cfg.flags |= flags;
+    if (!measurement && cfg.mode == TOMO_MODE_REF_COMPAT) {
+        try {
+            const std::vector<float>& m = packaged_measurement(preset);
+            measurement = m.data();
+            n_measurement = m.size();
+        } catch (const std::exception&) {
+            return TOMO_E_INVALID;
+        }
+    }
+    return tomo_eval(c, &cfg, params, n_params, measurement, n_measurement, fitness_out, nullptr, nullptr);
 }
 
-double FDTD_CPU(const std::vector<double>& arguments) {
-    return run(TOMO_PRESET_REF_CPU, arguments, "TOMO_MEASUREMENT_CPU", "measurement_cpu.f32", false, " fitness = ");
-}
+double FDTD_GPU(const std::vector<double>& arguments) { return run(TOMO_PRESET_REF_GPU, arguments, "fitness is: "); }
+
+double FDTD_CPU(const std::vector<double>& arguments) { return run(TOMO_PRESET_REF_CPU, arguments, " fitness = "); }

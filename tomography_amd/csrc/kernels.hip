@@ -1047,23 +1047,29 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     kcfloat* srcw = (kcfloat*)Sp->srcw;
     const int y0 = tile.y0, y1 = tile.y1;
     const int step0 = it.step0 + ordinal * T;
+    // Row addresses are wave-uniform (scalar base + row * pitch); lanes add
+    // their 32-bit byte offset, so loads and stores use the scalar-base
+    // addressing form instead of per-lane 64-bit address arithmetic.
+    const unsigned bx = (unsigned)x * 4u;  // x >= 0 on interior tiles
+    typedef GAS char gbyte;
+    auto at = [&](gfloat* base, unsigned off) -> gfloat& { return *(gfloat*)((gbyte*)base + off); };
     auto load = [&](int r, ICell& c) {
-        const int b = r * pitch + x;
-        c.ez = fin[b];
-        c.hx = fin[2 * ps + b];
-        c.hy = fin[4 * ps + b];
-        c.mat = mat[b];
+        gfloat* row = fin + (size_t)r * pitch;
+        c.ez = at(row, bx);
+        c.hx = at(row + 2 * (size_t)ps, bx);
+        c.hy = at(row + 4 * (size_t)ps, bx);
+        c.mat = (mat + (size_t)r * pitch)[(unsigned)x];
         c.jp = 0.f;
         const int lo = span[3 * r], hi = span[3 * r + 1];
-        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        if (x >= lo && x < hi) c.jp = at(jin + span[3 * r + 2] - lo, bx);
     };
     auto store = [&](int yo, const ICell& c) {
-        const int b = yo * pitch + x;
-        fout[b] = c.ez;
-        fout[2 * ps + b] = c.hx;
-        fout[4 * ps + b] = c.hy;
+        gfloat* row = fout + (size_t)yo * pitch;
+        at(row, bx) = c.ez;
+        at(row + 2 * (size_t)ps, bx) = c.hx;
+        at(row + 4 * (size_t)ps, bx) = c.hy;
         const int lo = span[3 * yo], hi = span[3 * yo + 1];
-        if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+        if (x >= lo && x < hi) at(jout + span[3 * yo + 2] - lo, bx) = c.jp;
     };
     ICell cur[T];
     float hxp[T];

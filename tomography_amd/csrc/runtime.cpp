@@ -817,9 +817,10 @@ static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* para
     }
     if (!rc) {
         const int terms = (tph ? 1 : 0) | ((cfg.flags & TOMO_FLAG_REL_FITNESS) ? 2 : 0);
+        const double norm = (cfg.flags & TOMO_FLAG_FITNESS_PER_EXC) ? (double)E : (double)E * F;
         LaunchTimer lt(c);
-        hipError_t e = launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, (int)n_vec, (double)E * F,
-                                      terms, dF, c->stream);
+        hipError_t e = launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, (int)n_vec, norm, terms, dF,
+                                      c->stream);
         lt.done(TOMO_KERNEL_FITNESS);
         if (e != hipSuccess) rc = fail(TOMO_E_HIP, hipGetErrorString(e));
     }
