@@ -182,6 +182,13 @@ typedef struct tomo_stats {
     tomo_kernel_stats update; /* launches = epochs */
 } tomo_stats;
 int tomo_profile(tomo_ctx *ctx, int enable); /* (re)starts the counters */
+/*
+ * Replay an epoch whose simulations all advance a full DFT chunk from one
+ * captured hipGraph (default off: measured no faster than direct launches).
+ * Profiling launches directly.  The graph is
+ * re-captured whenever the set of resident simulations or the schedule changes.
+ */
+int tomo_use_graphs(tomo_ctx *ctx, int enable);
 int tomo_get_stats(tomo_ctx *ctx, tomo_stats *out);
 
 /* ---- evaluation -------------------------------------------------------- */

@@ -326,3 +326,34 @@ def test_objective_entry_matches_eval(ctx):
     f_rel = ctx.fitness(t.PRESET_REF_GPU, p, flags=t.FLAG_REL_FITNESS)
     f_com = ctx.fitness(t.PRESET_REF_GPU, p, flags=t.FLAG_REL_FITNESS | t.FLAG_FITNESS_PER_EXC)
     assert abs(f_com - f_rel * cfg.n_freq) <= 1e-12 * abs(f_com)
+
+
+@pytest.mark.parametrize("form", [t.FORM_STREAMS, t.FORM_FUSED])
+def test_graph_replay_equals_direct_launches(form):
+    """Epochs replayed from the captured hipGraph (several replays, with host
+    reads of the fields between them, and a re-capture when a simulation
+    joins) give the same fields, accumulators and fitness as launching every
+    kernel directly."""
+    cfg = phys_small(steps=256)
+    out = []
+    for graphs in (False, True):
+        c = t.Context()
+        c.set_schedule(4, 16, 2, 32, 4, 8, form, 4)
+        c.use_graphs(graphs)
+        a = c.sim(cfg, PHYS_PARAMS, exc=0)
+        snaps = []
+        for _ in range(4):
+            a.step(64)  # two full chunks: two replays of one graph
+            snaps.append([f.copy() for f in a.fields()])
+        b = c.sim(cfg, PHYS_PARAMS + [1, 1, 0.5, 1, 2, 1, 0.1, 0], exc=1)
+        b.step(96)
+        snaps.append([f.copy() for f in b.fields()])
+        snaps.append(list(a.dft()) + list(b.dft()))
+        vecs = np.array([PHYS_PARAMS, PHYS_PARAMS - [2, 0, 1, 2, 3, 2, 0.1, 0]])
+        fb, Db = c.eval_batch(cfg, vecs, want_farfield=True)
+        snaps.append([np.asarray(fb), np.asarray(Db)])
+        out.append(snaps)
+        a.close(); b.close(); c.close()
+    for sa, sb in zip(*out):
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y)

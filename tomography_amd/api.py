@@ -24,7 +24,7 @@ EXPORTS = [
     "tomo_last_error", "tomo_set_schedule", "tomo_get_schedule", "tomo_autotune", "tomo_eval", "tomo_eval_batch",
     "tomo_target_farfield", "tomo_submit", "tomo_poll", "tomo_wait", "tomo_cancel", "tomo_progress",
     "tomo_request_bytes", "tomo_sim_create", "tomo_sim_destroy", "tomo_sim_step", "tomo_sim_fields",
-    "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_get_stats",
+    "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_use_graphs", "tomo_get_stats",
     "tomo_search_run", "tomo_fitness",
 ]
 
@@ -129,6 +129,7 @@ def lib():
     L.tomo_sim_dft.argtypes = [vp, _fp, _fp]
     L.tomo_sim_farfield.argtypes = [vp, _fp, _fp]
     L.tomo_profile.argtypes = [vp, _c.c_int]
+    L.tomo_use_graphs.argtypes = [vp, _c.c_int]
     L.tomo_get_stats.argtypes = [vp, _c.c_void_p]
     L.tomo_search_run.argtypes = [vp, vp, _dp, _dp, vp, _dp, vp]
     L.tomo_fitness.argtypes = [vp, _c.c_int, _c.c_int32, _dp, _c.c_size_t, _fp, _c.c_size_t, _dp]
@@ -191,6 +192,10 @@ class Context:
     def profile(self, enable=True):
         """Start (or stop) per-launch HIP-event timing; resets the counters."""
         _check(lib().tomo_profile(self.h, int(enable)))
+
+    def use_graphs(self, enable=True):
+        """Replay full epochs from a captured hipGraph (default) or launch each kernel."""
+        _check(lib().tomo_use_graphs(self.h, int(enable)))
 
     def stats(self):
         st = Stats()

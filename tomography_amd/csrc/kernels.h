@@ -10,6 +10,11 @@ namespace tomo {
 // Steps fused per launch that have a compiled kernel.
 bool tblock_supported(int T);
 
+// Time the next update, fold, projection or fitness launch made by the calling
+// thread: its dispatch packet carries `start` and `stop` (hipExtLaunchKernelGGL),
+// so timing puts no marker packets between kernels.  (nullptr, nullptr) disarms.
+void arm_launch_events(hipEvent_t start, hipEvent_t stop);
+
 // Time-blocked E/H update of every tile in `tiles` (T steps, one launch):
 // the general kernel (CPML, contour samples, edges, ref-compat physics) and
 // the interior kernel (physical mode, tiles away from every boundary).

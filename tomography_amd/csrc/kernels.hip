@@ -19,6 +19,7 @@
 // rows above and below: stage t is exact on lanes [t+1, 63-t) and the final
 // stage on [T, 64-T).  State is ping-ponged between two buffers per launch so
 // that overlapping ghost reads never see a neighbour's update.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -1500,16 +1501,35 @@ __global__ void k_zero(uint4* __restrict__ p, size_t n16) {
 
 bool tblock_supported(int T) { return T == 1 || T == 2 || T == 4 || T == 6 || T == 8; }
 
+namespace {
+// Events armed for the next timed launch of this thread (see arm_launch_events).
+thread_local hipEvent_t t_start = nullptr, t_stop = nullptr;
+}  // namespace
+
+void arm_launch_events(hipEvent_t start, hipEvent_t stop) {
+    t_start = start;
+    t_stop = stop;
+}
+
+// A launch whose dispatch packet carries the armed start/stop events (none
+// when nothing is armed); the events are consumed by this launch.
+#define TIMED_LAUNCH(K, G, B, SHM, ST, ...)                                        \
+    do {                                                                           \
+        hipEvent_t ea_ = t_start, eb_ = t_stop;                                    \
+        t_start = t_stop = nullptr;                                                \
+        hipExtLaunchKernelGGL(K, G, B, SHM, ST, ea_, eb_, 0u, __VA_ARGS__);        \
+    } while (0)
+
 hipError_t launch_tblock(int T, int mode, const TileItem* tiles, int ntiles, const LaunchItem* items,
                          const DevSim* sims, int ordinal, int waves, int grid, hipStream_t st) {
     dim3 g(grid), b(waves * kWave);
 #define TB(TT)                                                                                                 \
     if (T == TT) {                                                                                             \
         if (mode == TOMO_MODE_REF_COMPAT)                                                                      \
-            hipLaunchKernelGGL((k_tblock<TT, TOMO_MODE_REF_COMPAT>), g, b, 0, st, tiles, ntiles, items, sims,   \
+            TIMED_LAUNCH((k_tblock<TT, TOMO_MODE_REF_COMPAT>), g, b, 0, st, tiles, ntiles, items, sims,   \
                                ordinal);                                                                       \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_tblock<TT, TOMO_MODE_PHYSICAL>), g, b, 0, st, tiles, ntiles, items, sims,     \
+            TIMED_LAUNCH((k_tblock<TT, TOMO_MODE_PHYSICAL>), g, b, 0, st, tiles, ntiles, items, sims,     \
                                ordinal);                                                                       \
         return hipGetLastError();                                                                              \
     }
@@ -1523,7 +1543,7 @@ hipError_t launch_tedge(int T, const TileItem* tiles, int ntiles, const LaunchIt
     dim3 g(grid), b(waves * kWave);
 #define TE(TT)                                                                                       \
     if (T == TT) {                                                                                   \
-        hipLaunchKernelGGL((k_tedge<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);         \
+        TIMED_LAUNCH((k_tedge<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);         \
         return hipGetLastError();                                                                    \
     }
     TE(1) TE(2) TE(4) TE(6) TE(8)
@@ -1536,7 +1556,7 @@ hipError_t launch_trow(int T, const TileItem* tiles, int ntiles, const LaunchIte
     dim3 g(grid), b(waves * kWave);
 #define TR(TT)                                                                                       \
     if (T == TT) {                                                                                   \
-        hipLaunchKernelGGL((k_trow<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
+        TIMED_LAUNCH((k_trow<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
         return hipGetLastError();                                                                    \
     }
     TR(1) TR(2) TR(4) TR(6) TR(8)
@@ -1549,7 +1569,7 @@ hipError_t launch_tcol(int T, const TileItem* tiles, int ntiles, const LaunchIte
     dim3 g(grid), b(waves * kWave);
 #define TC(TT)                                                                                       \
     if (T == TT) {                                                                                   \
-        hipLaunchKernelGGL((k_tcol<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
+        TIMED_LAUNCH((k_tcol<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);          \
         return hipGetLastError();                                                                    \
     }
     TC(1) TC(2) TC(4) TC(6) TC(8)
@@ -1562,7 +1582,7 @@ hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchI
     dim3 g(grid), b(waves * kWave);
 #define TI(TT)                                                                                       \
     if (T == TT) {                                                                                   \
-        hipLaunchKernelGGL((k_tinner<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
+        TIMED_LAUNCH((k_tinner<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
         return hipGetLastError();                                                                    \
     }
     TI(1) TI(2) TI(4) TI(6) TI(8)
@@ -1575,7 +1595,7 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
     dim3 g(grid), b(waves * kWave);
 #define TF(TT)                                                                                       \
     if (T == TT) {                                                                                   \
-        hipLaunchKernelGGL((k_tfused<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
+        TIMED_LAUNCH((k_tfused<TT>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);        \
         return hipGetLastError();                                                                    \
     }
     TF(1) TF(2) TF(4) TF(6) TF(8)
@@ -1587,21 +1607,21 @@ hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const
                                int n_angles, float eta0, float* D, float* Pph, hipStream_t st) {
     dim3 g((n_angles + kProjAngles - 1) / kProjAngles, (n_sims + 8 * kProjWaves - 1) / (8 * kProjWaves), n_freq),
         b(64 * kProjWaves);
-    hipLaunchKernelGGL(k_project_mfma, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
+    TIMED_LAUNCH(k_project_mfma, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
     return hipGetLastError();
 }
 
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
                        hipStream_t st) {
     dim3 g((max_p + 255) / 256, nitems), b(256);
-    if (fp64) hipLaunchKernelGGL((k_fold<true>), g, b, 0, st, items, sims);
-    else hipLaunchKernelGGL((k_fold<false>), g, b, 0, st, items, sims);
+    if (fp64) TIMED_LAUNCH((k_fold<true>), g, b, 0, st, items, sims);
+    else TIMED_LAUNCH((k_fold<false>), g, b, 0, st, items, sims);
     return hipGetLastError();
 }
 
 hipError_t launch_fitness(const float* D, const float* Pph, const float* tgt, const float* tph, int L, int n_eval,
                           double norm, int phase_term, double* fit, hipStream_t st) {
-    hipLaunchKernelGGL(k_fitness, dim3(n_eval), dim3(256), 0, st, D, Pph, tgt, tph, L, norm, phase_term, fit);
+    TIMED_LAUNCH(k_fitness, dim3(n_eval), dim3(256), 0, st, D, Pph, tgt, tph, L, norm, phase_term, fit);
     return hipGetLastError();
 }
 

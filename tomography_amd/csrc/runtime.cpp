@@ -102,7 +102,10 @@ struct tomo_ctx {
     int grid_mult = 4;                 // persistent workgroups per CU
     int form = TOMO_FORM_STREAMS;      // launch form of physical-mode blocks
     int ER = 8;                        // output rows per corner / row-edge tile
-    bool use_graph = true;
+    bool use_graph = false;            // replay regular epochs from a captured hipGraph
+    hipGraphExec_t gexec = nullptr;    // the captured epoch and what it was captured for
+    std::vector<uint64_t> gkey_serials;
+    std::vector<int64_t> gkey;
     // DevSim table
     DevSim* d_sims = nullptr;
     int cap = 0;
@@ -142,7 +145,8 @@ struct tomo_ctx {
     bool prof = false;
     tomo_stats stats{};
     std::vector<hipEvent_t> ev_pool;
-    struct Pending { hipEvent_t a, b; int kind; double cells, bytes; };  // kind: TOMO_KERNEL_*, or -1 = update
+    // kind: TOMO_KERNEL_*, or -1 = update phase (borrows the events of its first and last launches)
+    struct Pending { hipEvent_t a, b; int kind; double cells, bytes; bool owns = true; };
     std::vector<Pending> pending;
 };
 
@@ -451,7 +455,8 @@ hipEvent_t pool_event(tomo_ctx* c) {
         return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing only: no system-scope fence (cache write-back) when the event completes
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -468,29 +473,34 @@ int drain_timings(tomo_ctx* c) {
         k.ms += ms;
         k.cell_steps += p.cells;
         k.alg_bytes += p.bytes;
-        c->ev_pool.push_back(p.a);
-        c->ev_pool.push_back(p.b);
     }
+    for (auto& p : c->pending)
+        if (p.owns) {
+            c->ev_pool.push_back(p.a);
+            c->ev_pool.push_back(p.b);
+        }
     c->pending.clear();
     return TOMO_OK;
 }
 
+// Times the one kernel launched between construction and done(): the launch's
+// dispatch packet carries the two events, so profiling inserts no marker
+// packets between kernels (a marker per launch slowed a 1024^2 evaluation by
+// about 12%).
 struct LaunchTimer {
     tomo_ctx* c;
-    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    LaunchTimer(tomo_ctx* c_, hipStream_t s_ = nullptr) : c(c_), st(s_ ? s_ : c_->stream) {
+    explicit LaunchTimer(tomo_ctx* c_) : c(c_) {
         if (c->prof) {
             a = pool_event(c);
             b = pool_event(c);
-            if (a) (void)hipEventRecord(a, st);
+            if (a && b) arm_launch_events(a, b);
         }
     }
+    ~LaunchTimer() { arm_launch_events(nullptr, nullptr); }
+    // call only after the launch succeeded: its events are then recorded
     void done(int kind, double cells = 0, double bytes = 0) {
-        if (c->prof && a && b) {
-            (void)hipEventRecord(b, st);
-            c->pending.push_back({a, b, kind, cells, bytes});
-        }
+        if (c->prof && a && b) c->pending.push_back({a, b, kind, cells, bytes, true});
     }
 };
 
@@ -573,66 +583,111 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     bytes += nst * ms[i]->cls_bytes[k];
                 }
             };
-            LaunchTimer whole(c, c->stream);
-            if (multi) HIPCHK(hipEventRecord(c->ev_bar, c->stream));
-            double ucells = 0, ubytes = 0;
-            for (int o = 0; o < nl; ++o) {
-                double cells[4], bytes[4];
-                for (int k = 0; k < 4; ++k) {
-                    block_work(o, k, cells[k], bytes[k]);
-                    ucells += cells[k];
-                    ubytes += bytes[k];
+            // the launches of one epoch: ceil(nsteps / T) time blocks, then the fold
+            auto launch_epoch = [&]() -> int {
+                if (multi) HIPCHK(hipEventRecord(c->ev_bar, c->stream));
+                double ucells = 0, ubytes = 0;
+                hipEvent_t first = nullptr;  // start of the update phase: its first launch
+                for (int o = 0; o < nl; ++o) {
+                    double cells[4], bytes[4];
+                    for (int k = 0; k < 4; ++k) {
+                        block_work(o, k, cells[k], bytes[k]);
+                        ucells += cells[k];
+                        ubytes += bytes[k];
+                    }
+                    if (mode != TOMO_MODE_PHYSICAL) {
+                        LaunchTimer lt(c);
+                        HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
+                                             std::min(n_edge, cap), c->stream));
+                        lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
+                        if (!first) first = lt.a;
+                        continue;
+                    }
+                    if (fused) {
+                        const int nt = c->ntiles;
+                        LaunchTimer lt(c);
+                        HIPCHK(launch_tfused(T, c->d_tiles, nt, c->d_items, c->d_sims, o, std::max(W, c->Wi),
+                                             std::min(nt, 2 * cap), c->stream));
+                        lt.done(TOMO_KERNEL_FUSED, cells[0] + cells[1] + cells[2] + cells[3],
+                                bytes[0] + bytes[1] + bytes[2] + bytes[3]);
+                        if (!first) first = lt.a;
+                        continue;
+                    }
+                    // every class of block o starts after all of block o-1 (ev_bar)
+                    int off = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        const int n = cnt[k];
+                        const TileItem* tl = c->d_tiles + off;
+                        off += n;
+                        if (n == 0) continue;
+                        hipStream_t es = multi ? c->side[k] : c->stream;
+                        if (multi) HIPCHK(hipStreamWaitEvent(es, c->ev_bar, 0));
+                        LaunchTimer lt(c);
+                        if (k == 0) HIPCHK(launch_tedge(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                        if (k == 1) HIPCHK(launch_trow(T, tl, n, c->d_items, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
+                        if (k == 2) HIPCHK(launch_tcol(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                        lt.done(TOMO_KERNEL_CORNER + k, cells[k], bytes[k]);
+                        if (!first) first = lt.a;
+                        if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
+                    }
+                    if (n_inner > 0) {
+                        LaunchTimer lt(c);
+                        HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, c->d_items, c->d_sims, o, c->Wi,
+                                             std::min(n_inner, 2 * cap), c->stream));
+                        lt.done(TOMO_KERNEL_INNER, cells[3], bytes[3]);
+                        if (!first) first = lt.a;
+                    }
+                    if (multi) {
+                        for (int k = 0; k < 3; ++k)
+                            if (cnt[k] > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side[k], 0));
+                        HIPCHK(hipEventRecord(c->ev_bar, c->stream));
+                    }
                 }
-                if (mode != TOMO_MODE_PHYSICAL) {
-                    LaunchTimer lt(c, c->stream);
-                    HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
-                                         std::min(n_edge, cap), c->stream));
-                    lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
-                    continue;
+                // the fold runs after every update launch of the epoch: its start ends the update phase
+                hipEvent_t fold_start = nullptr;
+                for (int prec = 0; prec < 2; ++prec) {
+                    if (!(prec ? fp64_any : fp32_any)) continue;
+                    LaunchTimer lt(c);
+                    HIPCHK(launch_fold(prec == 1, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+                    lt.done(TOMO_KERNEL_FOLD);
+                    if (!fold_start) fold_start = lt.a;
                 }
-                if (fused) {
-                    const int nt = c->ntiles;
-                    LaunchTimer lt(c, c->stream);
-                    HIPCHK(launch_tfused(T, c->d_tiles, nt, c->d_items, c->d_sims, o, std::max(W, c->Wi),
-                                         std::min(nt, 2 * cap), c->stream));
-                    lt.done(TOMO_KERNEL_FUSED, cells[0] + cells[1] + cells[2] + cells[3],
-                            bytes[0] + bytes[1] + bytes[2] + bytes[3]);
-                    continue;
+                if (c->prof && first && fold_start) c->pending.push_back({first, fold_start, -1, ucells, ubytes, false});
+                return TOMO_OK;
+            };
+            // A full epoch of an unchanged set of simulations is captured once into a
+            // hipGraph and replayed (timed runs launch directly: their per-launch
+            // events cannot live inside a graph).
+            int minn = K;
+            for (size_t i = 0; i < ms.size(); ++i) minn = std::min(minn, c->h_items[i].nsteps);
+            if (c->use_graph && !c->prof && minn == K) {
+                const std::vector<int64_t> gk = {mode, T, S, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
+                                                 fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)c->d_items,
+                                                 (int64_t)(intptr_t)c->d_tiles, (int64_t)(intptr_t)c->d_sims};
+                if (!c->gexec || gk != c->gkey || key != c->gkey_serials) {
+                    if (c->gexec) {
+                        HIPCHK(hipGraphExecDestroy(c->gexec));
+                        c->gexec = nullptr;
+                    }
+                    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+                    const int rc2 = launch_epoch();
+                    hipGraph_t g = nullptr;
+                    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+                    if (rc2) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc2;
+                    }
+                    HIPCHK(ec);
+                    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    HIPCHK(ei);
+                    c->gkey = gk;
+                    c->gkey_serials = key;
                 }
-                // every class of block o starts after all of block o-1 (ev_bar)
-                int off = 0;
-                for (int k = 0; k < 3; ++k) {
-                    const int n = cnt[k];
-                    const TileItem* tl = c->d_tiles + off;
-                    off += n;
-                    if (n == 0) continue;
-                    hipStream_t es = multi ? c->side[k] : c->stream;
-                    if (multi) HIPCHK(hipStreamWaitEvent(es, c->ev_bar, 0));
-                    LaunchTimer lt(c, es);
-                    if (k == 0) HIPCHK(launch_tedge(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
-                    if (k == 1) HIPCHK(launch_trow(T, tl, n, c->d_items, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
-                    if (k == 2) HIPCHK(launch_tcol(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
-                    lt.done(TOMO_KERNEL_CORNER + k, cells[k], bytes[k]);
-                    if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
-                }
-                if (n_inner > 0) {
-                    LaunchTimer lt(c, c->stream);
-                    HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, c->d_items, c->d_sims, o, c->Wi,
-                                         std::min(n_inner, 2 * cap), c->stream));
-                    lt.done(TOMO_KERNEL_INNER, cells[3], bytes[3]);
-                }
-                if (multi) {
-                    for (int k = 0; k < 3; ++k)
-                        if (cnt[k] > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side[k], 0));
-                    HIPCHK(hipEventRecord(c->ev_bar, c->stream));
-                }
-            }
-            whole.done(-1, ucells, ubytes);
-            {
-                LaunchTimer lt(c);
-                if (fp32_any) HIPCHK(launch_fold(false, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
-                if (fp64_any) HIPCHK(launch_fold(true, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
-                lt.done(TOMO_KERNEL_FOLD);
+                HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            } else {
+                const int rc2 = launch_epoch();
+                if (rc2) return rc2;
             }
             if (c->pending.size() > 4096) {
                 int rc2 = drain_timings(c);
@@ -722,7 +777,7 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
         LaunchTimer lt(c);
         e = launch_project_mfma(d_idx + g.i0, (int)g.n, cf.n_freq, c->d_sims, d_k + kofs[gi], cf.n_angles,
                                 (float)eta0(), dD + out_off[g.i0], dP ? dP + out_off[g.i0] : nullptr, c->stream);
-        lt.done(TOMO_KERNEL_PROJECT);
+        if (e == hipSuccess) lt.done(TOMO_KERNEL_PROJECT);
     }
     // the uploads read host vectors of this frame
     const hipError_t es = hipStreamSynchronize(c->stream);
@@ -821,7 +876,7 @@ static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* para
         LaunchTimer lt(c);
         hipError_t e = launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, (int)n_vec, norm, terms, dF,
                                       c->stream);
-        lt.done(TOMO_KERNEL_FITNESS);
+        if (e == hipSuccess) lt.done(TOMO_KERNEL_FITNESS);
         if (e != hipSuccess) rc = fail(TOMO_E_HIP, hipGetErrorString(e));
     }
     if (!rc) {
@@ -1022,9 +1077,10 @@ int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx** out) {
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int k = 0; k < 3; ++k) {
         HIPCHK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c->ev_side[k], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_side[k], hipEventDisableTiming | hipEventDisableSystemFence));
     }
-    HIPCHK(hipEventCreateWithFlags(&c->ev_bar, hipEventDisableTiming));
+    // ordering between this device's streams only: no system-scope fence needed
+    HIPCHK(hipEventCreateWithFlags(&c->ev_bar, hipEventDisableTiming | hipEventDisableSystemFence));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -1054,6 +1110,7 @@ int tomo_ctx_destroy(tomo_ctx* c) {
     if (c->h_tiles) (void)hipHostFree(c->h_tiles);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     if (c->staged) (void)hipEventDestroy(c->staged);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     (void)drain_timings(c);
     release_cache(c);
     for (int k = 0; k < 3; ++k) {
@@ -1187,6 +1244,13 @@ int tomo_profile(tomo_ctx* c, int enable) {
     c->prof = enable != 0;
     c->stats = tomo_stats{};
     return rc;
+}
+
+int tomo_use_graphs(tomo_ctx* c, int enable) {
+    if (!c) return fail(TOMO_E_INVALID, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    c->use_graph = enable != 0;
+    return TOMO_OK;
 }
 
 int tomo_get_stats(tomo_ctx* c, tomo_stats* out) {
