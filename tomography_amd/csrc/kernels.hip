@@ -35,6 +35,8 @@ constexpr int kWave = 64;
 // already run near their register budget, and their throughput in large
 // batches depends on occupancy more than on prefetch depth.
 constexpr int kGroupInner = 4, kGroupRow = 1, kGroupCol = 1, kGroupEdge = 1, kGroupRef = 2;
+// Prefetch groups of the fused kernel (one or a few simulations, latency-bound).
+constexpr int kFusedGroupInner = 4, kFusedGroupRow = 1, kFusedGroupCol = 1, kFusedGroupEdge = 1;
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding).
 constexpr int kWaitVm0 = 0x0F70;
 
@@ -224,6 +226,51 @@ __device__ __forceinline__ void store_cell(const SimView& S, const float4* __res
 
 // LDS CPML table layout: [0] ke [1] km [2] ike [3] ikm [4] ae [5] be [6] am [7] bm, each TOMO_MAX_NC.
 #define CP(k, i) lcp[(k) * TOMO_MAX_NC + (i)]
+
+// Packed y-strip tables of the physical-mode row and corner tiles: E entries
+// {1/kappa_e, a_e, b_e, exists} at [0, kYTab) and H entries {1/kappa_m, a_m,
+// b_m, exists} at [kYTab, 2 kYTab).  Entry i < nc is strip cell i; kYID is
+// the identity of rows outside the strips and kYIDZ the identity of rows
+// without nodes of that field (PEC rows, rows outside the grid).  One 16-byte
+// LDS read per field and stage, and the node masks arrive as data, so the
+// stage is branch-free.
+constexpr int kYID = TOMO_MAX_NC - 1, kYIDZ = TOMO_MAX_NC, kYTab = TOMO_MAX_NC + 1;
+
+// Stage form of the row and corner tiles.  Deep time blocks (T = 8, the
+// single-simulation schedule) leave few waves per SIMD, so a tile's serial
+// latency sets the launch time: their stages use the packed tables and
+// arithmetic node masks, and write NTFF samples after the stage loop, so each
+// row step is one basic block whose LDS reads overlap across stages (corner
+// and row-edge tiles alone: 58 -> 42 us per 1024^2 launch).  Shallower
+// blocks (the batched schedule, throughput-bound) keep the select form,
+// which issues fewer instructions.
+template <int T>
+constexpr bool kStageBranchFree = T >= 8;
+// Minimum resident workgroups per CU of the fused kernel, which caps its
+// VGPRs: two for the branch-free corner stage (three would spill it to
+// scratch), three otherwise.
+template <int T>
+constexpr int kFusedWgPerCu = kStageBranchFree<T> ? 2 : 3;
+
+__device__ __forceinline__ void fill_ytab(const DevSim* __restrict__ Sp, float4* __restrict__ ly) {
+    for (int i = threadIdx.x; i < kYTab; i += blockDim.x) {
+        const float ok = i == kYIDZ ? 0.f : 1.f;
+        const bool id = i >= Sp->nc;
+        ly[i] = id ? make_float4(1.f, 0.f, 0.f, ok) : make_float4(Sp->ike[i], Sp->ae[i], Sp->be[i], 1.f);
+        ly[kYTab + i] = id ? make_float4(1.f, 0.f, 0.f, ok) : make_float4(Sp->ikm[i], Sp->am[i], Sp->bm[i], 1.f);
+    }
+}
+
+// Table index of a row at distance d from the outermost node row of its
+// field (d < 0: no node): strip cell nc-1-d inside the strip, else kYID.
+__device__ __forceinline__ int ytab_idx(int d, int nc) {
+    const int e = nc - 1 - d;
+    return e < 0 ? kYID : (e >= nc ? kYIDZ : e);
+}
+// Ez nodes are rows 1..ny-1 and H nodes rows 0..ny-1; the two strips of a
+// field never overlap (ny >= 2 nc + 4), so the nearer edge decides.
+__device__ __forceinline__ int ytab_e(int y, int ny, int nc) { return ytab_idx(min(y - 1, ny - 1 - y), nc); }
+__device__ __forceinline__ int ytab_h(int y, int ny, int nc) { return ytab_idx(min(y, ny - 1 - y), nc); }
 
 // One leapfrog step of one row: c holds row y at level n (fields Ez^n,
 // H^{n-1/2}); ez_up = Ez^n(x, y+1); hx_prev = Hx^{n+1/2}(x, y-1) on entry and
@@ -503,7 +550,9 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
 // compute exactly what k_tinner computes (multiplications by 1 and additions
 // of 0 are exact).
 template <int T>
-__device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcp) {
+__device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcp,
+                                          float4* __restrict__ ly) {
+    if constexpr (kStageBranchFree<T>) fill_ytab(Sp, ly);
     {
         float* lcf = (float*)lc4;
         for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
@@ -519,10 +568,11 @@ __device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4*
     }
 }
 
-template <int T>
+template <int T, int G>
 __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
-                                         const float4* __restrict__ lc4, const float* __restrict__ lcp) {
+                                         const float4* __restrict__ lc4, const float* __restrict__ lcp,
+                                         const float4* __restrict__ ly) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     constexpr int V = kWave - 2 * T;
@@ -552,6 +602,7 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
     const float kxe = CP(2, ixe), axe = CP(4, ixe), bxe = CP(5, ixe);
     const float kxh = CP(3, ixh), axh = CP(6, ixh), bxh = CP(7, ixh);
     const bool hx_ok = (x >= 0 && x < nx), ex_ok = (x >= 1 && x <= nx - 1);
+    const float hxf = hx_ok ? 1.f : 0.f, exf = ex_ok ? 1.f : 0.f;
     const int y0 = tile.y0, y1 = tile.y1;
     const int step0 = it.step0 + ordinal * T;
     const int k0 = ordinal * T;
@@ -572,49 +623,94 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             Cell c = cur[t];
-            if (FAST || t < nst) {
-                // wave-uniform row coefficients
-                const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
-                const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
+            if constexpr (!kStageBranchFree<T>) {
+                if (FAST || t < nst) {
+                    // wave-uniform row coefficients
+                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
+                    // H half-step with the CPML recursions
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    c.phx = bxh * c.phx + axh * dEx;
+                    c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+                    const float ty = dEx * kxh + S.dxf * c.phx;
+                    const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
+                    const bool hok = hx_ok && hy_ok;
+                    const float hy = hok ? c.hy + S.chy * ty : 0.f;
+                    const float hx = hok ? c.hx - S.chx * tx : 0.f;
+                    // E half-step with the CPML recursions and the Debye current
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    c.pex = bxe * c.pex + axe * dHy;
+                    c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+                    const float cy = dHy * kxe + S.dxf * c.pex;
+                    const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
+                    if (!FAST && wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
+                    const bool eok = ex_ok && ey_ok;
+                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                    c.ez = eok ? en : 0.f;
+                    c.hx = hx;
+                    c.hy = hy;
+                    if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
+                }
+            } else if (FAST || t < nst) {
+                // row coefficients and node masks (packed y tables)
+                const float4 ch = ly[kYTab + ytab_h(y, ny, nc)], ce = ly[ytab_e(y, ny, nc)];
                 // H half-step with the CPML recursions
                 const float dEx = lane_next(c.ez) - c.ez;
                 const float dEy = in.ez - c.ez;
                 c.phx = bxh * c.phx + axh * dEx;
-                c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+                c.phy = ch.z * c.phy + ch.y * dEy;
                 const float ty = dEx * kxh + S.dxf * c.phx;
-                const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
-                const bool hok = hx_ok && hy_ok;
-                const float hy = hok ? c.hy + S.chy * ty : 0.f;
-                const float hx = hok ? c.hx - S.chx * tx : 0.f;
+                const float tx = dEy * ch.x + S.dyf * c.phy;
+                // node masks as factors (0 or 1): exact for the nodes that
+                // exist, and no uniform branch splits the stage
+                const float hm = hxf * ch.w, em = exf * ce.w;
+                const float hy = (c.hy + S.chy * ty) * hm;
+                const float hx = (c.hx - S.chx * tx) * hm;
                 // E half-step with the CPML recursions and the Debye current
                 const float dHy = hy - lane_prev(hy);
                 const float dHx = hx - hxp[t];
                 hxp[t] = hx;
                 c.pex = bxe * c.pex + axe * dHy;
-                c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+                c.pey = ce.z * c.pey + ce.y * dHx;
                 const float cy = dHy * kxe + S.dxf * c.pex;
-                const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+                const float cx = dHx * ce.x + S.dyf * c.pey;
+                // without an Ez node: Ez stays 0 and Jp keeps its value
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
-                if (!FAST && wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
-                const bool eok = ex_ok && ey_ok;
-                c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
-                c.ez = eok ? en : 0.f;
+                const float ce_ = qa.x * em, ch_ = qa.y * em, cj_ = qb.x * em;
+                const float kp_ = qb.y * em + (1.f - em), bp_ = qb.z * em;
+                float en = ce_ * c.ez + ch_ * cy - ch_ * cx - cj_ * c.jp;
+                if (!FAST && wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t] * em;
+                c.jp = kp_ * c.jp + bp_ * (en - c.ez);
+                c.ez = en;
                 c.hx = hx;
                 c.hy = hy;
-                if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
             }
             cur[t] = in;
             in = c;
+        }
+        // branch-free form: NTFF samples of every stage after the stage
+        // arithmetic; stage t's output row is now cur[t + 1], the last stage's `in`
+        if constexpr (kStageBranchFree<T>) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                if ((FAST || t < nst) && y >= y0 && y < y1 && xown)
+                    write_samples(S, wf, t + 1 < T ? cur[t + 1] : in, x, y, k0 + t);
+            }
         }
         return in;
     };
     auto step_fast = [&](int r, Cell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, Cell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wf.src && S.sy >= y0 - 2 * T && S.sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step_fast);
-    else row_stream<T, kGroupEdge, Cell>(y0, y1, xown, load, store, step_gen);
+    if (nst == T && !src_near) row_stream<T, G, Cell>(y0, y1, xown, load, store, step_fast);
+    else row_stream<T, G, Cell>(y0, y1, xown, load, store, step_gen);
 }
 
 template <int T>
@@ -623,6 +719,7 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
                                                const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     __shared__ float lcp[8 * TOMO_MAX_NC];
+    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -630,10 +727,10 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        fill_edge<T>(Sp, lc4, lcp);
+        fill_edge<T>(Sp, lc4, lcp, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_edge<T>(tile, it, nst, Sp, ordinal, lc4, lcp);
+        body_edge<T, kGroupEdge>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
     }
 }
 
@@ -651,26 +748,32 @@ struct YCell {
 };
 
 template <int T>
-__device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcy) {
+__device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcy,
+                                         float4* __restrict__ ly) {
     {
         float* lcf = (float*)lc4;
         for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
     }
-    for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
-        const bool id = (i >= Sp->nc);
-        lcy[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
-        lcy[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
-        lcy[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
-        lcy[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
-        lcy[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
-        lcy[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+    if constexpr (kStageBranchFree<T>) {
+        fill_ytab(Sp, ly);
+    } else {
+        for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
+            const bool id = (i >= Sp->nc);
+            lcy[0 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ike[i];
+            lcy[1 * TOMO_MAX_NC + i] = id ? 1.f : Sp->ikm[i];
+            lcy[2 * TOMO_MAX_NC + i] = id ? 0.f : Sp->ae[i];
+            lcy[3 * TOMO_MAX_NC + i] = id ? 0.f : Sp->be[i];
+            lcy[4 * TOMO_MAX_NC + i] = id ? 0.f : Sp->am[i];
+            lcy[5 * TOMO_MAX_NC + i] = id ? 0.f : Sp->bm[i];
+        }
     }
 }
 
-template <int T>
+template <int T, int G>
 __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
-                                         const float4* __restrict__ lc4, const float* __restrict__ lcy) {
+                                         const float4* __restrict__ lc4, const float* __restrict__ lcy,
+                                         const float4* __restrict__ ly) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     constexpr int V = kWave - 2 * T;
@@ -741,6 +844,20 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
             if (yo < nc) ph_out[yo * pitch + x] = c.phy;
             else if (yo >= ny - nc && yo < ny) ph_out[(nc + yo - (ny - nc)) * pitch + x] = c.phy;
     };
+    // y-face NTFF samples of contour row y at step k of the chunk
+    auto row_sample = [&](const YCell& o, int y, int k) {
+        if (y == D) {
+            samp[k * P + x - D - 1] = o.hx;
+            samp[(K + k) * P + x - D - 1] = o.ez;
+        } else if (y == D + 1) {
+            samp[(2 * K + k) * P + x - D - 1] = o.ez;
+        } else if (y == ny - D - 1) {
+            samp[k * P + L2 + (cx_hi - x)] = o.hx;
+            samp[(K + k) * P + L2 + (cx_hi - x)] = o.ez;
+        } else {
+            samp[(2 * K + k) * P + L2 + (cx_hi - x)] = o.ez;
+        }
+    };
     // FAST: every stage active (nst == T) and no source cell in reach of the tile,
     // so the stage loop carries no per-stage branch.
     auto stepg = [&](auto fast, int r, YCell in) {
@@ -749,53 +866,81 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             YCell c = cur[t];
-            if (FAST || t < nst) {
-                const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
-                const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
+            if constexpr (!kStageBranchFree<T>) {
+                if (FAST || t < nst) {
+                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
+                    const float dEx = lane_next(c.ez) - c.ez;
+                    const float dEy = in.ez - c.ez;
+                    c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
+                    const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
+                    const float hy = hok ? c.hy + chy * dEx : 0.f;
+                    const float hx = hok ? c.hx - chx * tx : 0.f;
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp[t];
+                    hxp[t] = hx;
+                    c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
+                    const float cx = dHx * lcy[iye] + dyf * c.pey;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
+                    if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
+                    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+                    c.ez = eok ? en : 0.f;
+                    c.hx = hx;
+                    c.hy = hy;
+                    if ((y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 && y < y1 && own && xr)
+                        row_sample(c, y, k0 + t);
+                }
+            } else if (FAST || t < nst) {
+                // row coefficients and node masks (packed y tables)
+                const float4 ch = ly[kYTab + ytab_h(y, ny, nc)], ce = ly[ytab_e(y, ny, nc)];
+                // node masks as factors (0 or 1): exact for the nodes that
+                // exist, and no uniform branch splits the stage
+                const float hm = ch.w, em = ce.w;
                 const float dEx = lane_next(c.ez) - c.ez;
                 const float dEy = in.ez - c.ez;
-                c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
-                const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
-                const float hy = hok ? c.hy + chy * dEx : 0.f;
-                const float hx = hok ? c.hx - chx * tx : 0.f;
+                c.phy = ch.z * c.phy + ch.y * dEy;
+                const float tx = dEy * ch.x + dyf * c.phy;
+                const float hy = (c.hy + chy * dEx) * hm;
+                const float hx = (c.hx - chx * tx) * hm;
                 const float dHy = hy - lane_prev(hy);
                 const float dHx = hx - hxp[t];
                 hxp[t] = hx;
-                c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
-                const float cx = dHx * lcy[iye] + dyf * c.pey;
+                c.pey = ce.z * c.pey + ce.y * dHx;
+                const float cx = dHx * ce.x + dyf * c.pey;
+                // without an Ez node: Ez stays 0 and Jp keeps its value
                 const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
-                float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
-                if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
-                c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
-                c.ez = eok ? en : 0.f;
+                const float ce_ = qa.x * em, ch_ = qa.y * em, cj_ = qb.x * em;
+                const float kp_ = qb.y * em + (1.f - em), bp_ = qb.z * em;
+                float en = ce_ * c.ez + ch_ * dHy - ch_ * cx - cj_ * c.jp;
+                if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t] * em;
+                c.jp = kp_ * c.jp + bp_ * (en - c.ez);
+                c.ez = en;
                 c.hx = hx;
                 c.hy = hy;
-                if ((y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 && y < y1 && own && xr) {
-                    const int k = k0 + t;
-                    if (y == D) {
-                        samp[k * P + x - D - 1] = c.hx;
-                        samp[(K + k) * P + x - D - 1] = c.ez;
-                    } else if (y == D + 1) {
-                        samp[(2 * K + k) * P + x - D - 1] = c.ez;
-                    } else if (y == ny - D - 1) {
-                        samp[k * P + L2 + (cx_hi - x)] = c.hx;
-                        samp[(K + k) * P + L2 + (cx_hi - x)] = c.ez;
-                    } else {
-                        samp[(2 * K + k) * P + L2 + (cx_hi - x)] = c.ez;
-                    }
-                }
             }
             cur[t] = in;
             in = c;
+        }
+        // branch-free form: y-face NTFF samples after the stage arithmetic;
+        // stage t's output row is now cur[t + 1], the last stage's `in`
+        if constexpr (kStageBranchFree<T>) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                if ((FAST || t < nst) && (y == D || y == D + 1 || y == ny - D - 1 || y == ny - D) && y >= y0 &&
+                    y < y1 && own && xr)
+                    row_sample(t + 1 < T ? cur[t + 1] : in, y, k0 + t);
+            }
         }
         return in;
     };
     auto step_fast = [&](int r, YCell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, YCell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, kGroupRow, YCell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_stream<T, G, YCell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, G, YCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -805,6 +950,7 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
                                                const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     __shared__ float lcy[6 * TOMO_MAX_NC];
+    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -812,10 +958,10 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        fill_row<T>(Sp, lc4, lcy);
+        fill_row<T>(Sp, lc4, lcy, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_row<T>(tile, it, nst, Sp, ordinal, lc4, lcy);
+        body_row<T, kGroupRow>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
     }
 }
 
@@ -847,7 +993,7 @@ __device__ __forceinline__ void fill_col(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T>
+template <int T, int G>
 __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcx) {
@@ -957,31 +1103,41 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
                 c.ez = eok ? en : 0.f;
                 c.hx = hx;
                 c.hy = hy;
-                if (wcc && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
-                    const int k = k0 + t;
-                    if (x == nx - D - 1) {
-                        samp[k * P + L1 + y - cy_lo] = c.hy;
-                        samp[(K + k) * P + L1 + y - cy_lo] = c.ez;
-                    } else if (x == nx - D) {
-                        samp[(2 * K + k) * P + L1 + y - cy_lo] = c.ez;
-                    } else if (x == D) {
-                        samp[k * P + L3 + (ny - D - 1 - y)] = c.hy;
-                        samp[(K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
-                    } else {
-                        samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = c.ez;
-                    }
-                }
             }
             cur[t] = in;
             in = c;
+        }
+        // x-face NTFF samples after the stage arithmetic (which thus stays one
+        // basic block): stage t's output row is now cur[t + 1], the last
+        // stage's is `in`.
+        if (wcc) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int y = r - 1 - t;
+                if ((FAST || t < nst) && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
+                    const XCell& o = t + 1 < T ? cur[t + 1] : in;
+                    const int k = k0 + t;
+                    if (x == nx - D - 1) {
+                        samp[k * P + L1 + y - cy_lo] = o.hy;
+                        samp[(K + k) * P + L1 + y - cy_lo] = o.ez;
+                    } else if (x == nx - D) {
+                        samp[(2 * K + k) * P + L1 + y - cy_lo] = o.ez;
+                    } else if (x == D) {
+                        samp[k * P + L3 + (ny - D - 1 - y)] = o.hy;
+                        samp[(K + k) * P + L3 + (ny - D - 1 - y)] = o.ez;
+                    } else {
+                        samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = o.ez;
+                    }
+                }
+            }
         }
         return in;
     };
     auto step_fast = [&](int r, XCell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, XCell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, kGroupCol, XCell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_stream<T, G, XCell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, G, XCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1001,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         fill_col<T>(Sp, lc4, lcx);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_col<T>(tile, it, nst, Sp, ordinal, lc4, lcx);
+        body_col<T, kGroupCol>(tile, it, nst, Sp, ordinal, lc4, lcx);
     }
 }
 
@@ -1024,7 +1180,7 @@ __device__ __forceinline__ void fill_inner(const DevSim* __restrict__ Sp, float4
     }
 }
 
-template <int T>
+template <int T, int G>
 __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4) {
@@ -1114,8 +1270,8 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     auto step_fast = [&](int r, ICell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, ICell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, kGroupInner, ICell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_stream<T, G, ICell>(y0, y1, own, load, store, step_fast);
+    else row_stream<T, G, ICell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1134,7 +1290,7 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
         fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_inner<T>(tile, it, nst, Sp, ordinal, lc4);
+        body_inner<T, kGroupInner>(tile, it, nst, Sp, ordinal, lc4);
     }
 }
 
@@ -1143,15 +1299,16 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
 // class in TileItem::cls).  One launch per time block with no cross-stream
 // joins: the form for few simulations, where every block's latency is on the
 // critical path.  Classes are listed slowest first so their tiles start first.
-// Three workgroups of 256 threads per CU (<= 168 VGPRs): the corner body would
-// otherwise hold the whole kernel to two, and a 1024^2 grid's tiles would no
-// longer start in one wave of workgroups.
+// Workgroups per CU: kFusedWgPerCu (three for the select-form stages, so a
+// 1024^2 grid's tiles start in one wave of workgroups; two for the
+// branch-free T = 8 stages, whose corner body needs the registers).
 template <int T>
-__global__ __launch_bounds__(256, 3) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
                                                 const LaunchItem* __restrict__ items,
                                                 const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
     __shared__ float ltab[8 * TOMO_MAX_NC];
+    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -1160,16 +1317,16 @@ __global__ __launch_bounds__(256, 3) void k_tfused(const TileItem* __restrict__ 
         const DevSim* Sp = sims + it.sim;
         const int cls = tile.cls;
         __syncthreads();
-        if (cls == TILE_CORNER) fill_edge<T>(Sp, lc4, ltab);
-        else if (cls == TILE_ROW) fill_row<T>(Sp, lc4, ltab);
+        if (cls == TILE_CORNER) fill_edge<T>(Sp, lc4, ltab, ly);
+        else if (cls == TILE_ROW) fill_row<T>(Sp, lc4, ltab, ly);
         else if (cls == TILE_COL) fill_col<T>(Sp, lc4, ltab);
         else fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        if (cls == TILE_CORNER) body_edge<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
-        else if (cls == TILE_ROW) body_row<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
-        else if (cls == TILE_COL) body_col<T>(tile, it, nst, Sp, ordinal, lc4, ltab);
-        else body_inner<T>(tile, it, nst, Sp, ordinal, lc4);
+        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_COL) body_col<T, kFusedGroupCol>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else body_inner<T, kFusedGroupInner>(tile, it, nst, Sp, ordinal, lc4);
     }
 }
 
