@@ -33,7 +33,7 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
                          int ordinal, int waves, int grid, hipStream_t st);
 
 // Fold one chunk of surface samples into the running DFT (Kahan FP32 or FP64).
-hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
+hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p, int max_f,
                        hipStream_t st);
 
 // Far-field projection of a group of simulations that share grid, contour and

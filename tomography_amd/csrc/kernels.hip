@@ -779,7 +779,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
     constexpr int V = kWave - 2 * T;
     constexpr int ID = TOMO_MAX_NC - 1;
     const int par = it.parity0 ^ (ordinal & 1);
-    const int ps = Sp->ps, js = Sp->js, ys = Sp->ys, pitch = Sp->pitch, nx = Sp->nx, ny = Sp->ny, nc = Sp->nc;
+    const int ps = Sp->ps, js = Sp->js, ys = Sp->ys, pitch = Sp->pitch, ny = Sp->ny, nc = Sp->nc;
     gfloat* fin = (gfloat*)Sp->fbase + par * ps;
     gfloat* fout = (gfloat*)Sp->fbase + (par ^ 1) * ps;
     gfloat* jin = (gfloat*)Sp->jbase + par * js;
@@ -1334,16 +1334,22 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
 // Phase e^{-j w (step + t0) dt}: evaluated in FP64 for every step of the chunk
 // (<= 64 steps) into an LDS table and rounded once to FP32; the chunk is summed
 // in FP32 and added to the long-running accumulator with Kahan compensation.
-constexpr int kFoldGroup = 8;  // frequencies per pass of the fold
-
-template <bool FP64>
+// Block (x, item, z) folds contour points [256 x, 256 x + 256) of one item
+// for frequencies [z fpb, z fpb + fpb), kFoldGroup at a time.  Few items
+// (one simulation) take fpb = kFoldGroup = 2, so the frequencies spread over
+// more blocks; many items take fpb = all frequencies with kFoldGroup = 8, so
+// each sample is read once per 8 frequencies.  Every frequency's sum runs in
+// the same order either way.
+template <bool FP64, int kFoldGroup>
 __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ items,
-                                              const DevSim* __restrict__ sims) {
+                                              const DevSim* __restrict__ sims, int fpb) {
     const LaunchItem it = items[blockIdx.y];
     const DevSim* S = sims + it.sim;
-    const int P = S->P, F = S->nfreq, K = S->chunk;
+    const int P = S->P, K = S->chunk;
+    const int fb = blockIdx.z * fpb, F = min(S->nfreq, fb + fpb);
     const int nk = min(it.nsteps, K);
-    if (nk <= 0 || ((S->flags & TOMO_FLAG_DFT_FP64) != 0) != FP64) return;
+    if (nk <= 0 || fb >= F || ((S->flags & TOMO_FLAG_DFT_FP64) != 0) != FP64) return;
+    const int nf = F - fb;
     __shared__ float2 wJ[FP64 ? 1 : TOMO_MAX_FREQ * 64];
     __shared__ float2 wM[FP64 ? 1 : TOMO_MAX_FREQ * 64];
     __shared__ double2 dJ[FP64 ? TOMO_MAX_FREQ * 64 : 1];
@@ -1351,8 +1357,8 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
     if (!FP64) {
         // every phase of the chunk evaluated in FP64 and rounded once to FP32
         // (a rotation recurrence would drift by an ulp per step)
-        for (int i = threadIdx.x; i < 2 * F * nk; i += blockDim.x) {
-            const int which = i / (F * nk), f = (i / nk) % F, k = i % nk;
+        for (int i = threadIdx.x; i < 2 * nf * nk; i += blockDim.x) {
+            const int which = i / (nf * nk), f = fb + (i / nk) % nf, k = i % nk;
             double s0, c0;
             sincos(-S->wdt[f] * ((double)it.step0 + k + (which ? S->tM : S->tJ)), &s0, &c0);
             (which ? wM : wJ)[f * 64 + k] = make_float2((float)c0, (float)s0);
@@ -1360,16 +1366,16 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
     } else {
         // FP64 study mode: exact phases; the M phase is the J phase rotated by
         // -w (tM - tJ).
-        for (int i = threadIdx.x; i < F * nk; i += blockDim.x) {
-            const int f = i / nk, k = i % nk;
+        for (int i = threadIdx.x; i < nf * nk; i += blockDim.x) {
+            const int f = fb + i / nk, k = i % nk;
             double s, c;
             sincos(-S->wdt[f] * ((double)it.step0 + k + S->tJ), &s, &c);
             dJ[f * 64 + k] = make_double2(c, s);
         }
-        if (threadIdx.x < F) {
+        if (threadIdx.x < nf) {
             double s, c;
-            sincos(-S->wdt[threadIdx.x] * (S->tM - S->tJ), &s, &c);
-            dRot[threadIdx.x] = make_double2(c, s);
+            sincos(-S->wdt[fb + threadIdx.x] * (S->tM - S->tJ), &s, &c);
+            dRot[fb + threadIdx.x] = make_double2(c, s);
         }
     }
     __syncthreads();
@@ -1377,16 +1383,21 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
     if (p >= P) return;
     const int leg = p < S->L1 ? 0 : (p < S->L2 ? 1 : (p < S->L3 ? 2 : 3));
     const float sdt = S->sgn_dt[leg];
-    const float* hs = S->samp + p;
-    const float* e0 = S->samp + (size_t)K * P + p;
-    const float* e1 = S->samp + (size_t)2 * K * P + p;
+    // global-address-space views (flat loads would also count against the LDS
+    // counter, so every phase-table wait would drain the sample loads)
+    gcfloat* hs = (gcfloat*)S->samp + p;
+    gcfloat* e0 = (gcfloat*)S->samp + (size_t)K * P + p;
+    gcfloat* e1 = (gcfloat*)S->samp + (size_t)2 * K * P + p;
     // Frequencies in groups of kFoldGroup: each sample is read once per group
     // (coalesced over p) and multiplied into every frequency of the group.
-    for (int f0 = 0; f0 < F; f0 += kFoldGroup) {
+    for (int f0 = fb; f0 < F; f0 += kFoldGroup) {
         if (!FP64) {
             float jr[kFoldGroup], ji[kFoldGroup], mr[kFoldGroup], mi[kFoldGroup];
 #pragma unroll
             for (int q = 0; q < kFoldGroup; ++q) jr[q] = ji[q] = mr[q] = mi[q] = 0.f;
+            // unrolled so that several steps' samples are in flight at once
+            // (the accumulation order is unchanged)
+#pragma unroll 8
             for (int k = 0; k < nk; ++k) {
                 const float h = hs[(size_t)k * P];
                 const float e = (e0[(size_t)k * P] + e1[(size_t)k * P]) / 2;
@@ -1418,6 +1429,7 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
             double jr[kFoldGroup], ji[kFoldGroup], mr[kFoldGroup], mi[kFoldGroup];
 #pragma unroll
             for (int q = 0; q < kFoldGroup; ++q) jr[q] = ji[q] = mr[q] = mi[q] = 0.0;
+#pragma unroll 8
             for (int k = 0; k < nk; ++k) {
                 const double h = hs[(size_t)k * P];
                 const double e = 0.5 * ((double)e0[(size_t)k * P] + (double)e1[(size_t)k * P]);
@@ -1768,11 +1780,20 @@ hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const
     return hipGetLastError();
 }
 
-hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p,
+hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p, int max_f,
                        hipStream_t st) {
-    dim3 g((max_p + 255) / 256, nitems), b(256);
-    if (fp64) TIMED_LAUNCH((k_fold<true>), g, b, 0, st, items, sims);
-    else TIMED_LAUNCH((k_fold<false>), g, b, 0, st, items, sims);
+    const int bx = (max_p + 255) / 256;
+    // fewer than about one block per CU: spread the frequencies too
+    const bool narrow = bx * nitems < 256;
+    const int fpb = narrow ? 2 : max_f;
+    dim3 g(bx, nitems, (max_f + fpb - 1) / fpb), b(256);
+    if (fp64) {
+        if (narrow) TIMED_LAUNCH((k_fold<true, 2>), g, b, 0, st, items, sims, fpb);
+        else TIMED_LAUNCH((k_fold<true, 8>), g, b, 0, st, items, sims, fpb);
+    } else {
+        if (narrow) TIMED_LAUNCH((k_fold<false, 2>), g, b, 0, st, items, sims, fpb);
+        else TIMED_LAUNCH((k_fold<false, 8>), g, b, 0, st, items, sims, fpb);
+    }
     return hipGetLastError();
 }
 

@@ -532,7 +532,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             int rc = ensure_staging(c, (int)ms.size(), ntiles);
             if (rc) return rc;
             HIPCHK(hipEventSynchronize(c->staged));
-            int maxP = 0, maxn = 0;
+            int maxP = 0, maxF = 0, maxn = 0;
             for (size_t i = 0; i < ms.size(); ++i) {
                 Slot* s = ms[i];
                 LaunchItem& it = c->h_items[i];
@@ -541,6 +541,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 it.parity0 = s->parity;
                 it.nsteps = std::min(K, s->target - s->done);
                 maxP = std::max(maxP, s->d.P);
+                maxF = std::max(maxF, s->d.nfreq);
                 maxn = std::max(maxn, it.nsteps);
             }
             HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * ms.size(), hipMemcpyHostToDevice,
@@ -648,7 +649,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 for (int prec = 0; prec < 2; ++prec) {
                     if (!(prec ? fp64_any : fp32_any)) continue;
                     LaunchTimer lt(c);
-                    HIPCHK(launch_fold(prec == 1, c->d_items, (int)ms.size(), c->d_sims, maxP, c->stream));
+                    HIPCHK(launch_fold(prec == 1, c->d_items, (int)ms.size(), c->d_sims, maxP, maxF, c->stream));
                     lt.done(TOMO_KERNEL_FOLD);
                     if (!fold_start) fold_start = lt.a;
                 }
