@@ -1475,12 +1475,17 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
 // register r of lane l holds C[8(r/4) + 4(l/32) + r%4][l%32], so a lane ends
 // up holding all four rows of simulations 2q + l/32 (q = r/4) at angle l%32
 // and forms D and the phase without leaving its registers.
+// Few simulations (SPLIT): the four waves share 8 simulations and take one
+// 32-angle tile each, so a lone simulation keeps four waves busy instead of
+// one.  Every output element sees the same products in the same order in
+// both layouts, so the results are identical.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kProjAngleTiles = 4;                 // 32-angle tiles per workgroup
 constexpr int kProjAngles = 32 * kProjAngleTiles;
 constexpr int kProjChunk = 16;                     // contour points per LDS phase tile
 constexpr int kProjWaves = 4;                      // waves (8 simulations each) per workgroup
 
+template <bool SPLIT>
 __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __restrict__ sim_idx, int n_sims,
                                                                   const DevSim* __restrict__ sims,
                                                                   const float2* __restrict__ kinv, int A,
@@ -1489,7 +1494,9 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
     __shared__ float2 wt[kProjChunk][kProjAngles];  // phase tile {cos, sin}
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int a0 = blockIdx.x * kProjAngles, f = blockIdx.z;
-    const int s0 = (blockIdx.y * kProjWaves + wv) * 8;  // this wave's first simulation
+    constexpr int AT = SPLIT ? 1 : kProjAngleTiles;     // 32-angle tiles per wave
+    const int s0 = SPLIT ? blockIdx.y * 8 : (blockIdx.y * kProjWaves + wv) * 8;  // this wave's first simulation
+    const int mt0 = SPLIT ? wv : 0;                      // this wave's first angle tile
     const DevSim* G = sims + sim_idx[0];                 // geometry shared by the group
     const int P = G->P, L1 = G->L1, L2 = G->L2, L3 = G->L3, F = G->nfreq;
     const bool fp64 = (G->flags & TOMO_FLAG_DFT_FP64) != 0;
@@ -1510,9 +1517,9 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
         const DevSim* S = sims + sim_idx[sl];
         accX = rowk == 0 ? S->accJ : S->accM;
     }
-    f32x16 cre[kProjAngleTiles], cim[kProjAngleTiles];
+    f32x16 cre[AT], cim[AT];
 #pragma unroll
-    for (int m = 0; m < kProjAngleTiles; ++m)
+    for (int m = 0; m < AT; ++m)
 #pragma unroll
         for (int r = 0; r < 16; ++r) cre[m][r] = cim[m][r] = 0.f;
     for (int c0 = 0; c0 < P; c0 += kProjChunk) {
@@ -1547,8 +1554,8 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
 #pragma unroll
         for (int j = 0; j < kProjChunk / 2; ++j) {
 #pragma unroll
-            for (int m = 0; m < kProjAngleTiles; ++m) {
-                const float2 w = wt[2 * j + kh][32 * m + (lane & 31)];
+            for (int m = 0; m < AT; ++m) {
+                const float2 w = wt[2 * j + kh][32 * (mt0 + m) + (lane & 31)];
                 cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.x, cre[m], 0, 0, 0);
                 cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(-xi[j], w.y, cre[m], 0, 0, 0);
                 cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.y, cim[m], 0, 0, 0);
@@ -1558,8 +1565,8 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
     }
     const float inv = ki.y;
 #pragma unroll
-    for (int m = 0; m < kProjAngleTiles; ++m) {
-        const int ab = a0 + 32 * m + (lane & 31);
+    for (int m = 0; m < AT; ++m) {
+        const int ab = a0 + 32 * (mt0 + m) + (lane & 31);
         if (ab >= A) continue;
         float cph, sph;
         sincosf(6.283185307179586f * (float)ab / (float)A, &sph, &cph);
@@ -1774,9 +1781,11 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
 
 hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const DevSim* sims, const float2* kinv,
                                int n_angles, float eta0, float* D, float* Pph, hipStream_t st) {
-    dim3 g((n_angles + kProjAngles - 1) / kProjAngles, (n_sims + 8 * kProjWaves - 1) / (8 * kProjWaves), n_freq),
-        b(64 * kProjWaves);
-    TIMED_LAUNCH(k_project_mfma, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
+    const bool split = n_sims < 8 * kProjWaves;
+    const int per_wg = split ? 8 : 8 * kProjWaves;  // simulations per workgroup
+    dim3 g((n_angles + kProjAngles - 1) / kProjAngles, (n_sims + per_wg - 1) / per_wg, n_freq), b(64 * kProjWaves);
+    if (split) TIMED_LAUNCH(k_project_mfma<true>, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
+    else TIMED_LAUNCH(k_project_mfma<false>, g, b, 0, st, sim_idx, n_sims, sims, kinv, n_angles, eta0, D, Pph);
     return hipGetLastError();
 }
 
