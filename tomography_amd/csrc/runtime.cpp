@@ -504,6 +504,9 @@ struct LaunchTimer {
     }
 };
 
+// Epochs whose launch items one upload carries, and the item count it may hold.
+constexpr int kMaxStagedEpochs = 256, kMaxStagedItems = 65536;
+
 // Run epochs until every slot in `sl` reaches its target step.
 int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
     const int T = c->T, S = c->S, W = c->W, K = c->K;
@@ -529,22 +532,30 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     build_tiles(ms[i], (int)i, T, S, W, c->Wi, c->ER, te, tr, tc, tn);
             }
             const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : c->ntiles;
-            int rc = ensure_staging(c, (int)ms.size(), ntiles);
+            // Items of E consecutive epochs in one upload: every simulation of
+            // the set is active in each of them (only the last may be partial),
+            // so the set, and with it the tile list, stays fixed.  A captured
+            // graph reads one epoch's items at a fixed address, so graphs stage one.
+            const int nsim = (int)ms.size();
+            int E = c->use_graph ? 1 : std::max(1, std::min(kMaxStagedEpochs, kMaxStagedItems / nsim));
+            for (Slot* s : ms) E = std::min(E, (s->target - s->done + K - 1) / K);
+            int rc = ensure_staging(c, E * nsim, ntiles);
             if (rc) return rc;
             HIPCHK(hipEventSynchronize(c->staged));
-            int maxP = 0, maxF = 0, maxn = 0;
-            for (size_t i = 0; i < ms.size(); ++i) {
+            int maxP = 0, maxF = 0;
+            for (int i = 0; i < nsim; ++i) {
                 Slot* s = ms[i];
-                LaunchItem& it = c->h_items[i];
-                it.sim = s->idx;
-                it.step0 = s->done;
-                it.parity0 = s->parity;
-                it.nsteps = std::min(K, s->target - s->done);
+                for (int e = 0; e < E; ++e) {
+                    LaunchItem& it = c->h_items[e * nsim + i];
+                    it.sim = s->idx;
+                    it.step0 = s->done + e * K;
+                    it.parity0 = s->parity ^ ((e * (K / T)) & 1);  // epochs before the last are full
+                    it.nsteps = std::min(K, s->target - s->done - e * K);
+                }
                 maxP = std::max(maxP, s->d.P);
                 maxF = std::max(maxF, s->d.nfreq);
-                maxn = std::max(maxn, it.nsteps);
             }
-            HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * ms.size(), hipMemcpyHostToDevice,
+            HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * E * nsim, hipMemcpyHostToDevice,
                                   c->stream));
             if (rebuild) {
                 TileItem* h = c->h_tiles;
@@ -565,7 +576,10 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             HIPCHK(hipEventRecord(c->staged, c->stream));
             const int n_edge = c->n_edge, n_inner = c->ntiles - c->n_edge;
             const int cap = c->cus * c->grid_mult;
-            const int nl = (maxn + T - 1) / T;
+            // the staged epoch being launched: its items on the host and the device
+            const LaunchItem* h_it = c->h_items;
+            LaunchItem* d_it = c->d_items;
+            int nl = 0;  // time blocks of that epoch
             // tile classes: corners (general kernel), row-edge, column-edge, interior
             const int n0 = c->n_corner, nr = c->n_rowedge, ncl = n_edge - n0 - nr;
             const int cnt[3] = {n0, nr, ncl};
@@ -578,7 +592,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 cells = bytes = 0;
                 if (!c->prof) return;
                 for (size_t i = 0; i < ms.size(); ++i) {
-                    const int nst = std::min(T, c->h_items[i].nsteps - o * T);
+                    const int nst = std::min(T, h_it[i].nsteps - o * T);
                     if (nst <= 0) continue;
                     cells += nst * ms[i]->cls_cells[k];
                     bytes += nst * ms[i]->cls_bytes[k];
@@ -598,7 +612,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     }
                     if (mode != TOMO_MODE_PHYSICAL) {
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, c->d_items, c->d_sims, o, W,
+                        HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, d_it, c->d_sims, o, W,
                                              std::min(n_edge, cap), c->stream));
                         lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
                         if (!first) first = lt.a;
@@ -607,7 +621,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     if (fused) {
                         const int nt = c->ntiles;
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tfused(T, c->d_tiles, nt, c->d_items, c->d_sims, o, std::max(W, c->Wi),
+                        HIPCHK(launch_tfused(T, c->d_tiles, nt, d_it, c->d_sims, o, std::max(W, c->Wi),
                                              std::min(nt, 2 * cap), c->stream));
                         lt.done(TOMO_KERNEL_FUSED, cells[0] + cells[1] + cells[2] + cells[3],
                                 bytes[0] + bytes[1] + bytes[2] + bytes[3]);
@@ -624,16 +638,16 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                         hipStream_t es = multi ? c->side[k] : c->stream;
                         if (multi) HIPCHK(hipStreamWaitEvent(es, c->ev_bar, 0));
                         LaunchTimer lt(c);
-                        if (k == 0) HIPCHK(launch_tedge(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
-                        if (k == 1) HIPCHK(launch_trow(T, tl, n, c->d_items, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
-                        if (k == 2) HIPCHK(launch_tcol(T, tl, n, c->d_items, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                        if (k == 0) HIPCHK(launch_tedge(T, tl, n, d_it, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                        if (k == 1) HIPCHK(launch_trow(T, tl, n, d_it, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
+                        if (k == 2) HIPCHK(launch_tcol(T, tl, n, d_it, c->d_sims, o, W, std::min(n, 2 * cap), es));
                         lt.done(TOMO_KERNEL_CORNER + k, cells[k], bytes[k]);
                         if (!first) first = lt.a;
                         if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
                     }
                     if (n_inner > 0) {
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, c->d_items, c->d_sims, o, c->Wi,
+                        HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, d_it, c->d_sims, o, c->Wi,
                                              std::min(n_inner, 2 * cap), c->stream));
                         lt.done(TOMO_KERNEL_INNER, cells[3], bytes[3]);
                         if (!first) first = lt.a;
@@ -649,7 +663,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 for (int prec = 0; prec < 2; ++prec) {
                     if (!(prec ? fp64_any : fp32_any)) continue;
                     LaunchTimer lt(c);
-                    HIPCHK(launch_fold(prec == 1, c->d_items, (int)ms.size(), c->d_sims, maxP, maxF, c->stream));
+                    HIPCHK(launch_fold(prec == 1, d_it, (int)ms.size(), c->d_sims, maxP, maxF, c->stream));
                     lt.done(TOMO_KERNEL_FOLD);
                     if (!fold_start) fold_start = lt.a;
                 }
@@ -659,49 +673,58 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             // A full epoch of an unchanged set of simulations is captured once into a
             // hipGraph and replayed (timed runs launch directly: their per-launch
             // events cannot live inside a graph).
-            int minn = K;
-            for (size_t i = 0; i < ms.size(); ++i) minn = std::min(minn, c->h_items[i].nsteps);
-            if (c->use_graph && !c->prof && minn == K) {
-                const std::vector<int64_t> gk = {mode, T, S, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
-                                                 fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)c->d_items,
-                                                 (int64_t)(intptr_t)c->d_tiles, (int64_t)(intptr_t)c->d_sims};
-                if (!c->gexec || gk != c->gkey || key != c->gkey_serials) {
-                    if (c->gexec) {
-                        HIPCHK(hipGraphExecDestroy(c->gexec));
-                        c->gexec = nullptr;
-                    }
-                    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-                    const int rc2 = launch_epoch();
-                    hipGraph_t g = nullptr;
-                    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
-                    if (rc2) {
-                        if (g) (void)hipGraphDestroy(g);
-                        return rc2;
-                    }
-                    HIPCHK(ec);
-                    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    HIPCHK(ei);
-                    c->gkey = gk;
-                    c->gkey_serials = key;
+            for (int e = 0; e < E; ++e) {
+                h_it = c->h_items + e * nsim;
+                d_it = c->d_items + e * nsim;
+                int maxn = 0, minn = K;
+                for (int i = 0; i < nsim; ++i) {
+                    maxn = std::max(maxn, h_it[i].nsteps);
+                    minn = std::min(minn, h_it[i].nsteps);
                 }
-                HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-            } else {
-                const int rc2 = launch_epoch();
-                if (rc2) return rc2;
-            }
-            if (c->pending.size() > 4096) {
-                int rc2 = drain_timings(c);
-                if (rc2) return rc2;
-            }
+                nl = (maxn + T - 1) / T;
+                if (c->use_graph && !c->prof && minn == K) {
+                    const std::vector<int64_t> gk = {mode, T, S, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
+                                                     fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)d_it,
+                                                     (int64_t)(intptr_t)c->d_tiles, (int64_t)(intptr_t)c->d_sims};
+                    if (!c->gexec || gk != c->gkey || key != c->gkey_serials) {
+                        if (c->gexec) {
+                            HIPCHK(hipGraphExecDestroy(c->gexec));
+                            c->gexec = nullptr;
+                        }
+                        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+                        const int rc2 = launch_epoch();
+                        hipGraph_t g = nullptr;
+                        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+                        if (rc2) {
+                            if (g) (void)hipGraphDestroy(g);
+                            return rc2;
+                        }
+                        HIPCHK(ec);
+                        const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+                        (void)hipGraphDestroy(g);
+                        HIPCHK(ei);
+                        c->gkey = gk;
+                        c->gkey_serials = key;
+                    }
+                    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+                } else {
+                    const int rc2 = launch_epoch();
+                    if (rc2) return rc2;
+                }
+                if (c->pending.size() > 4096) {
+                    int rc2 = drain_timings(c);
+                    if (rc2) return rc2;
+                }
+                for (int i = 0; i < nsim; ++i) {
+                    Slot* s = ms[i];
+                    const int ns = h_it[i].nsteps;
+                    s->done += ns;
+                    s->parity ^= ((ns + T - 1) / T) & 1;
+                }
+            }  // staged epochs
             std::vector<Slot*> next;
-            for (size_t i = 0; i < ms.size(); ++i) {
-                Slot* s = ms[i];
-                const int n = c->h_items[i].nsteps;
-                s->done += n;
-                s->parity ^= ((n + T - 1) / T) & 1;
+            for (Slot* s : ms)
                 if (s->done < s->target) next.push_back(s);
-            }
             ms.swap(next);
         }
     }
