@@ -244,13 +244,17 @@ constexpr int kYID = TOMO_MAX_NC - 1, kYIDZ = TOMO_MAX_NC, kYTab = TOMO_MAX_NC +
 // and row-edge tiles alone: 58 -> 42 us per 1024^2 launch).  Shallower
 // blocks (the batched schedule, throughput-bound) keep the select form,
 // which issues fewer instructions.
+// kStageBranchFree: the form of the standalone row and corner kernels;
+// kFusedBranchFree: the form inside the fused kernel.
 template <int T>
 constexpr bool kStageBranchFree = T >= 8;
+template <int T>
+constexpr bool kFusedBranchFree = T >= 8;
 // Minimum resident workgroups per CU of the fused kernel, which caps its
 // VGPRs: two for the branch-free corner stage (three would spill it to
 // scratch), three otherwise.
 template <int T>
-constexpr int kFusedWgPerCu = kStageBranchFree<T> ? 2 : 3;
+constexpr int kFusedWgPerCu = kFusedBranchFree<T> ? 2 : 3;
 
 __device__ __forceinline__ void fill_ytab(const DevSim* __restrict__ Sp, float4* __restrict__ ly) {
     for (int i = threadIdx.x; i < kYTab; i += blockDim.x) {
@@ -549,10 +553,10 @@ __global__ __launch_bounds__(512) void k_tblock(const TileItem* __restrict__ til
 // registers, y-strip coefficients are wave-uniform per row.  Ordinary cells
 // compute exactly what k_tinner computes (multiplications by 1 and additions
 // of 0 are exact).
-template <int T>
+template <int T, bool BF>
 __device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcp,
                                           float4* __restrict__ ly) {
-    if constexpr (kStageBranchFree<T>) fill_ytab(Sp, ly);
+    if constexpr (BF) fill_ytab(Sp, ly);
     {
         float* lcf = (float*)lc4;
         for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
@@ -568,7 +572,7 @@ __device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4*
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool BF>
 __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcp,
@@ -623,7 +627,7 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             Cell c = cur[t];
-            if constexpr (!kStageBranchFree<T>) {
+            if constexpr (!BF) {
                 if (FAST || t < nst) {
                     // wave-uniform row coefficients
                     const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
@@ -696,7 +700,7 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         }
         // branch-free form: NTFF samples of every stage after the stage
         // arithmetic; stage t's output row is now cur[t + 1], the last stage's `in`
-        if constexpr (kStageBranchFree<T>) {
+        if constexpr (BF) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -718,8 +722,9 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
                                                const LaunchItem* __restrict__ items,
                                                const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    constexpr bool BF = kStageBranchFree<T>;
     __shared__ float lcp[8 * TOMO_MAX_NC];
-    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
+    __shared__ float4 ly[BF ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -727,10 +732,10 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        fill_edge<T>(Sp, lc4, lcp, ly);
+        fill_edge<T, BF>(Sp, lc4, lcp, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_edge<T, kGroupEdge>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
+        body_edge<T, kGroupEdge, BF>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
     }
 }
 
@@ -747,14 +752,14 @@ struct YCell {
     int mat;
 };
 
-template <int T>
+template <int T, bool BF>
 __device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* __restrict__ lc4, float* __restrict__ lcy,
                                          float4* __restrict__ ly) {
     {
         float* lcf = (float*)lc4;
         for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
     }
-    if constexpr (kStageBranchFree<T>) {
+    if constexpr (BF) {
         fill_ytab(Sp, ly);
     } else {
         for (int i = threadIdx.x; i < TOMO_MAX_NC; i += blockDim.x) {
@@ -769,7 +774,7 @@ __device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool BF>
 __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcy,
@@ -866,7 +871,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
         for (int t = 0; t < T; ++t) {
             const int y = r - 1 - t;
             YCell c = cur[t];
-            if constexpr (!kStageBranchFree<T>) {
+            if constexpr (!BF) {
                 if (FAST || t < nst) {
                     const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
                     const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
@@ -925,7 +930,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
         }
         // branch-free form: y-face NTFF samples after the stage arithmetic;
         // stage t's output row is now cur[t + 1], the last stage's `in`
-        if constexpr (kStageBranchFree<T>) {
+        if constexpr (BF) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int y = r - 1 - t;
@@ -949,8 +954,9 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
                                                const LaunchItem* __restrict__ items,
                                                const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    constexpr bool BF = kStageBranchFree<T>;
     __shared__ float lcy[6 * TOMO_MAX_NC];
-    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
+    __shared__ float4 ly[BF ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -958,10 +964,10 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         if (nst <= 0) continue;
         const DevSim* Sp = sims + it.sim;
         __syncthreads();
-        fill_row<T>(Sp, lc4, lcy, ly);
+        fill_row<T, BF>(Sp, lc4, lcy, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_row<T, kGroupRow>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
+        body_row<T, kGroupRow, BF>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
     }
 }
 
@@ -1307,8 +1313,9 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
                                                 const LaunchItem* __restrict__ items,
                                                 const DevSim* __restrict__ sims, int ordinal) {
     __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    constexpr bool BF = kFusedBranchFree<T>;
     __shared__ float ltab[8 * TOMO_MAX_NC];
-    __shared__ float4 ly[kStageBranchFree<T> ? 2 * kYTab : 1];
+    __shared__ float4 ly[BF ? 2 * kYTab : 1];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const TileItem tile = tiles[ti];
         const LaunchItem it = items[tile.item];
@@ -1317,14 +1324,14 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
         const DevSim* Sp = sims + it.sim;
         const int cls = tile.cls;
         __syncthreads();
-        if (cls == TILE_CORNER) fill_edge<T>(Sp, lc4, ltab, ly);
-        else if (cls == TILE_ROW) fill_row<T>(Sp, lc4, ltab, ly);
+        if (cls == TILE_CORNER) fill_edge<T, BF>(Sp, lc4, ltab, ly);
+        else if (cls == TILE_ROW) fill_row<T, BF>(Sp, lc4, ltab, ly);
         else if (cls == TILE_COL) fill_col<T>(Sp, lc4, ltab);
         else fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
-        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
         else if (cls == TILE_COL) body_col<T, kFusedGroupCol>(tile, it, nst, Sp, ordinal, lc4, ltab);
         else body_inner<T, kFusedGroupInner>(tile, it, nst, Sp, ordinal, lc4);
     }
