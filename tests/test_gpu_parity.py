@@ -357,3 +357,24 @@ def test_graph_replay_equals_direct_launches(form):
     for sa, sb in zip(*out):
         for x, y in zip(sa, sb):
             assert np.array_equal(x, y)
+
+
+def test_large_batch_layouts_equal_single_simulation_layouts(ctx):
+    """A batch large enough for the wide fold (8 frequencies per block) and
+    the 32-simulation projection layout gives the same far fields and fitness
+    as plain evaluations, which use the frequency-split fold and the
+    angle-split projection; many staged epochs (a 700-step run, K = 64) agree
+    with the same run stepped one epoch at a time by the request loop."""
+    cfg = phys_small(steps=700, n=96, n_exc=4).copy(n_freq=5, breast_radius=0.012, skin_thickness=0.001,
+                                                     src_radius=0.015)
+    rng = np.random.default_rng(7)
+    vecs = PHYS_PARAMS + rng.uniform(-1, 1, (40, 8)) * [2, 2, 0.5, 2, 3, 1, 0.1, 0]
+    vecs[:, 7] = 0
+    fb, Db = ctx.eval_batch(cfg, vecs, want_farfield=True)  # 160 simulations
+    for i in (0, 17, 39):
+        fp, Dp, _ = ctx.eval(cfg, vecs[i], want_farfield=True)
+        assert np.array_equal(Db[i], Dp)
+        assert fb[i] == fp
+    tk = ctx.submit(cfg, vecs[17])
+    f_served, _ = ctx.wait(tk)
+    assert f_served == fb[17]
