@@ -1225,21 +1225,33 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     const bool prof = c->prof;
     c->prof = false;
     tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form, c->ER};
+    // each trial times at least 512 steps: a few launches alone are dominated
+    // by first-launch and dispatch effects
+    const int epochs = std::max(2, (512 + c->K - 1) / c->K);
     double cells = 0;
-    for (Slot* s : sl) cells += (double)s->d.nx * s->d.ny * c->K;
+    for (Slot* s : sl) cells += (double)s->d.nx * s->d.ny * c->K * epochs;
     double best_t = 1e30;
     auto trial = [&](const tomo_schedule& cand) -> int {
         int r = set_schedule_locked(c, cand);
         if (r) return r;
         double t = 0;
-        r = time_epochs(c, sl, 2, &t);
+        r = time_epochs(c, sl, epochs, &t);
         if (r) return r;
         if (t < best_t) { best_t = t; cur = cand; }
         return TOMO_OK;
     };
     if (!rc) rc = trial(cur);
-    const int Ts[] = {1, 2, 4, 8}, Ss[] = {8, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
-    const int ERs[] = {2, 4, 8, 16};
+    // Seeds measured on MI355X, so that the descent does not start from a
+    // poor corner of the space: one simulation (fused, short tiles) and many
+    // (four streams, tall tiles).
+    if (tc.mode == TOMO_MODE_PHYSICAL) {
+        const tomo_schedule seeds[2] = {{4, 6, 4, 4, c->K, 16, TOMO_FORM_FUSED, 2},
+                                        {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 8}};
+        for (const tomo_schedule& sd : seeds)
+            if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
+    }
+    const int Ts[] = {1, 2, 4, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
+    const int ERs[] = {1, 2, 3, 4, 8, 16};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
@@ -1257,7 +1269,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     c->tile_key.clear();
     if (rc) return rc;
     if (best) *best = cur;
-    if (rate) *rate = cells * 2 / best_t;
+    if (rate) *rate = cells / best_t;
     return TOMO_OK;
 }
 
