@@ -42,3 +42,32 @@ def rel_l2(a, b):
     b = np.asarray(b).astype(np.complex128)
     n = np.sqrt((np.abs(b) ** 2).sum())
     return float(np.sqrt((np.abs(a - b) ** 2).sum()) / (n if n > 0 else 1.0))
+
+
+# A 200^2 grid (dx = 0.5 mm) filled with the Debye fat tissue and nothing
+# else: the breast outline lies outside the grid, no tumour block, and phantom
+# seed offset 126 puts all three fibroglandular ellipses outside the grid too
+# (the tests check the material map).  The antenna sits at the centre node.
+FAT = (3.1, 1.6, 13e-12)  # eps_inf, delta_eps, tau of the fat tissue (sigma = 0)
+FAT_OFFSET = 126
+
+
+def fat_medium(steps=4000):
+    c = phys_small(steps=steps, n=200, n_exc=1)
+    p = np.array([0.0, 0.0, 0.0, 20.0, 34.0, 13.0, 0.0, float(FAT_OFFSET)])
+    return c.copy(breast_radius=0.08, skin_thickness=0.0001, src_radius=0.0), p
+
+
+def line_source_ratio(f, d1, d2, dx, eps_inf, d_eps, tau):
+    """E(d2)/E(d1) of a time-harmonic line current in a homogeneous Debye medium:
+    H0^(2)(k d2) / H0^(2)(k d1), k = w sqrt(mu0 eps0 (eps_inf + d_eps / (1 + j w tau)))
+    (phasors e^{+jwt}, the convention of a DFT with kernel e^{-jwt})."""
+    from scipy.special import hankel2
+    w = 2 * np.pi * f
+    k = w * np.sqrt(4e-7 * np.pi * 8.854187817e-12 * (eps_inf + d_eps / (1 + 1j * w * tau)))
+    return hankel2(0, k * d2 * dx) / hankel2(0, k * d1 * dx)
+
+
+def spectrum(samples, f, dt_sample):
+    n = np.arange(len(samples))
+    return np.sum(np.asarray(samples, np.float64) * np.exp(-2j * np.pi * f * n * dt_sample))

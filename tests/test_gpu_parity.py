@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import tomography_amd as t
-from cases import PHYS_PARAMS, phys_small, ref_cpu_params, ref_cpu_small, ref_params, ref_small, rel_l2
+from cases import (FAT, PHYS_PARAMS, fat_medium, line_source_ratio, phys_small, ref_cpu_params, ref_cpu_small,
+                   ref_params, ref_small, rel_l2, spectrum)
 
 pytestmark = pytest.mark.gpu
 
@@ -378,3 +379,35 @@ def test_large_batch_layouts_equal_single_simulation_layouts(ctx):
     tk = ctx.submit(cfg, vecs[17])
     f_served, _ = ctx.wait(tk)
     assert f_served == fb[17]
+
+
+
+@pytest.mark.parametrize("form", [t.FORM_STREAMS, t.FORM_FUSED])
+def test_debye_medium_propagation_matches_hankel_closed_form(form):
+    """The HIP update itself against the closed form (no oracle): in a
+    homogeneous Debye medium the ratio of the field's spectra 40 and 20 cells
+    from the soft source is H0^(2)(40 k dx) / H0^(2)(20 k dx) with the Debye
+    eps(w), to 1e-3 in magnitude and 2e-3 rad, along both axes; the probes lie
+    in interior tiles and the source in an interior tile's row band, in both
+    launch forms."""
+    cfg, p = fat_medium()
+    c = t.Context()
+    c.set_schedule(tsteps=4, seg_rows=6, edge_waves=4, inner_waves=4, chunk=64, grid_mult=16, form=form, edge_rows=2)
+    g = c.sim(cfg, p, exc=0)
+    mat, _ = g.materials()
+    assert np.all(mat == 2)
+    sx = sy = cfg.nx // 2
+    d1, d2, every = 20, 40, 4
+    tr = []
+    for _ in range(cfg.steps // every):
+        g.step(every)
+        ez = g.fields()[0]
+        tr.append((ez[sy, sx + d1], ez[sy, sx + d2], ez[sy + d1, sx], ez[sy + d2, sx]))
+    tr = np.array(tr, np.float64)
+    for f in (1.5e9, 2.5e9, 3.5e9):
+        want = line_source_ratio(f, d1, d2, cfg.dx, *FAT)
+        for a, b in ((0, 1), (2, 3)):
+            got = spectrum(tr[:, b], f, every * cfg.dt) / spectrum(tr[:, a], f, every * cfg.dt)
+            assert abs(abs(got) / abs(want) - 1) < 1e-3, (f, got, want)
+            assert abs(np.angle(got / want)) < 2e-3, (f, got, want)
+    g.close(); c.close()

@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 import tomography_amd as t
-from cases import PHYS_PARAMS, phys_small, ref_cpu_params, ref_params, ref_small, rel_l2
+from cases import (FAT, PHYS_PARAMS, fat_medium, line_source_ratio, phys_small, ref_cpu_params, ref_params,
+                   ref_small, rel_l2, spectrum)
 from oracle.binding import Reference, unpack_ref_cj
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -310,3 +311,40 @@ def test_generic_fitness_normalisation_matches_fdtd_common(oracle):
     direct = -float(np.sum((target.astype(np.float64) - D) ** 2 / (cfg.n_exc * target.astype(np.float64) ** 2)))
     assert abs(f_com - direct) <= 1e-12 * abs(direct)
     assert abs(f_com - f_rel * cfg.n_freq) <= 1e-12 * abs(f_com)
+
+
+FAT_FREQS = (1.5e9, 2.0e9, 2.5e9, 3.0e9, 3.5e9, 4.0e9)
+FAT_MAG_TOL, FAT_PHASE_TOL = 1e-3, 2e-3  # relative |ratio|, radians
+
+
+def test_physical_debye_medium_propagation_matches_hankel_closed_form(oracle):
+    """Closed form for the whole physical-mode update (Yee leapfrog, Debye ADE,
+    soft source): in a homogeneous Debye medium a line current radiates
+    H0^(2)(k rho) with k = w sqrt(mu0 eps(w)), so the ratio of the field's
+    spectra 40 and 20 cells from the source depends only on eps(w).  The FP64
+    oracle meets it to 2e-4 (|ratio|) and 7e-4 rad; the same medium without its
+    dispersion (eps_inf alone, or the static eps_inf + delta_eps) misses by
+    1-9% and up to 0.3 rad, so a dropped or mis-signed polarisation current
+    fails here."""
+    cfg, p = fat_medium()
+    s = oracle.sim(cfg, p, exc=0, fp64=True)
+    mat, _ = s.materials()
+    assert np.all(mat == 2), "the grid must hold the fat tissue only"
+    sx, sy = oracle.source_cell(cfg, 0)
+    d1, d2, every = 20, 40, 4
+    tr = []
+    for _ in range(cfg.steps // every):
+        s.step(every)
+        tr.append((s.probe(sx + d1, sy), s.probe(sx + d2, sy), s.probe(sx, sy + d1), s.probe(sx, sy + d2)))
+    tr = np.array(tr)
+    assert np.abs(tr[-50:]).max() < 1e-4 * np.abs(tr).max()  # the pulse has passed both probes
+    for f in FAT_FREQS:
+        want = line_source_ratio(f, d1, d2, cfg.dx, *FAT)
+        for a, b in ((0, 1), (2, 3)):  # along x and along y
+            got = spectrum(tr[:, b], f, every * cfg.dt) / spectrum(tr[:, a], f, every * cfg.dt)
+            assert abs(abs(got) / abs(want) - 1) < FAT_MAG_TOL, (f, got, want)
+            assert abs(np.angle(got / want)) < FAT_PHASE_TOL, (f, got, want)
+        for wrong in ((FAT[0], 0.0, FAT[2]), (FAT[0] + FAT[1], 0.0, FAT[2])):
+            bad = line_source_ratio(f, d1, d2, cfg.dx, *wrong)
+            got = spectrum(tr[:, 1], f, every * cfg.dt) / spectrum(tr[:, 0], f, every * cfg.dt)
+            assert abs(abs(got) / abs(bad) - 1) > FAT_MAG_TOL or abs(np.angle(got / bad)) > FAT_PHASE_TOL
