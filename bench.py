@@ -415,10 +415,12 @@ def run_ref_compat(ctx, sched, n_eval=2, ref_steps=250):
             vecs = [np.concatenate([rng.uniform(1, 7, n2), rng.uniform(1.5, 41, n2), rng.uniform(0, 0.3, n2)])
                     for _ in range(n_eval + 1)]
         target = np.zeros(cfg.farfield_len(), np.float32)
-        ctx.eval(cfg, vecs[0], target=target)  # warm-up
+        # tomo_fitness: the call the FDTD_GPU / FDTD_CPU shims make (the
+        # vector's excitations advance together)
+        ctx.fitness(pid, vecs[0], measurement=target)  # warm-up
         t0 = time.perf_counter()
         for v in vecs[1:]:
-            ctx.eval(cfg, v, target=target)
+            ctx.fitness(pid, v, measurement=target)
         dt = (time.perf_counter() - t0) / n_eval
         out[name] = {"s_per_eval": round(dt, 4), "evals_per_s": round(1 / dt, 3),
                      "Mcell_steps_per_s": round(cell_steps(cfg) / dt / 1e6, 1)}

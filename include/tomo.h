@@ -207,9 +207,9 @@ int tomo_eval(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
 
 /*
  * Batched evaluation of n_vec parameter vectors (row-major, `dim` each) with
- * a shared configuration: all n_vec * n_exc simulations advance together in
- * time-blocked launches captured in a hipGraph.  Results equal tomo_eval's
- * for each vector to within FP32 reduction-order tolerance.
+ * a shared configuration: all n_vec * n_exc simulations (as many as the HBM
+ * budget admits at once) advance together in the same time-blocked launches.
+ * Results equal tomo_eval's for each vector bit for bit.
  */
 int tomo_eval_batch(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
                     size_t n_vec, size_t dim, const float *target, size_t n_target,

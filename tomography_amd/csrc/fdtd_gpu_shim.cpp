@@ -110,7 +110,9 @@ extern "C" int tomo_fitness(tomo_ctx* c, int preset, int32_t flags, const double
             return TOMO_E_INVALID;
         }
     }
-    return tomo_eval(c, &cfg, params, n_params, measurement, n_measurement, fitness_out, nullptr, nullptr);
+    // The vector's excitations are independent simulations: they advance
+    // together (the batched form, whose results equal tomo_eval's bit for bit).
+    return tomo_eval_batch(c, &cfg, params, 1, n_params, measurement, n_measurement, fitness_out, nullptr);
 }
 
 double FDTD_GPU(const std::vector<double>& arguments) { return run(TOMO_PRESET_REF_GPU, arguments, "fitness is: "); }
