@@ -1,6 +1,6 @@
 """HIP path (libtomo.so through the C ABI) against the oracle.
 
-Tolerances (written here, stated in DESIGN.md):
+Tolerances (written here):
   * material maps, coefficient tables, contour indexing: bit-exact;
   * fields after a few hundred steps: relative L2 <= 1e-5 against the FP32
     oracle's restatement of the reference, <= 1e-4 against the FP64 oracle;
