@@ -71,3 +71,50 @@ def line_source_ratio(f, d1, d2, dx, eps_inf, d_eps, tau):
 def spectrum(samples, f, dt_sample):
     n = np.arange(len(samples))
     return np.sum(np.asarray(samples, np.float64) * np.exp(-2j * np.pi * f * n * dt_sample))
+
+
+def debye_k(f, eps_inf, d_eps, tau):
+    w = 2 * np.pi * f
+    return w * np.sqrt(4e-7 * np.pi * 8.854187817e-12 * (eps_inf + d_eps / (1 + 1j * w * tau)))
+
+
+def cylinder_field(f, pos, centre, a, background, cylinder, n_terms=30):
+    """Ez (up to a constant factor) at `pos` (m, relative to a line current at
+    the origin) in a homogeneous Debye background holding a Debye cylinder of
+    radius a centred at `centre` (m), e^{+jwt} phasors:
+        E = H0^(2)(kb |r|) + sum_n a_n H_n^(2)(kb rho) e^{jn(phi - phi_s)},
+        a_n = -H_n^(2)(kb d) [kb J_n'(kb a) J_n(k1 a) - k1 J_n(kb a) J_n'(k1 a)]
+                            / [kb H_n^(2)'(kb a) J_n(k1 a) - k1 H_n^(2)(kb a) J_n'(k1 a)]
+    with (rho, phi) the probe and (d, phi_s) the source in cylinder-centred polar
+    coordinates (continuity of Ez and of dEz/drho at rho = a).  `cylinder` =
+    None gives the background alone."""
+    from scipy.special import h2vp, hankel2, jv, jvp
+    kb = debye_k(f, *background)
+    E = hankel2(0, kb * np.hypot(*pos))
+    if cylinder is None:
+        return E
+    k1 = debye_k(f, *cylinder)
+    rs = (-centre[0], -centre[1])
+    d, phs = np.hypot(*rs), np.arctan2(rs[1], rs[0])
+    r = (pos[0] - centre[0], pos[1] - centre[1])
+    rho, ph = np.hypot(*r), np.arctan2(r[1], r[0])
+    for n in range(-n_terms, n_terms + 1):
+        cn = (kb * jvp(n, kb * a) * jv(n, k1 * a) - k1 * jv(n, kb * a) * jvp(n, k1 * a)) / \
+             (kb * h2vp(n, kb * a) * jv(n, k1 * a) - k1 * hankel2(n, kb * a) * jvp(n, k1 * a))
+        E = E + (-hankel2(n, kb * d) * cn) * hankel2(n, kb * rho) * np.exp(1j * n * (ph - phs))
+    return E
+
+
+# A Debye cylinder (a tumour block: centre 10 mm from the antenna, radius 6 mm,
+# eps_inf 8, delta_eps 4, tau 13 ps, sigma 0) in the fat medium above, and the
+# probe offsets (cells from the antenna) around it: behind, beside, in front.
+CYL = (8.0, 4.0, 13e-12)
+CYL_PROBES = [(40, 0), (20, 24), (20, -24), (-20, 0), (30, 18), (50, 10)]
+CYL_REF = (0, -30)
+
+
+def fat_with_cylinder(steps=4000):
+    cfg, p = fat_medium(steps)
+    p = p.copy()
+    p[:7] = [10.0, 0.0, 6.0, CYL[0], CYL[1], CYL[2] * 1e12, 0.0]
+    return cfg, p
