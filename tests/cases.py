@@ -118,3 +118,33 @@ def fat_with_cylinder(steps=4000):
     p = p.copy()
     p[:7] = [10.0, 0.0, 6.0, CYL[0], CYL[1], CYL[2] * 1e12, 0.0]
     return cfg, p
+
+
+def skin_cylinder(steps=2000):
+    """Vacuum 128^2 grid holding one cylinder of the skin tissue (eps_r 36,
+    sigma 4 S/m): breast radius 6 mm, skin 5.9 mm (the interior shrinks to the
+    centre node), no tumour block; the antenna 16 mm from its centre."""
+    c = phys_small(steps=steps, n=128, n_exc=1)
+    p = PHYS_PARAMS.copy()
+    p[2] = 0.0
+    return c.copy(breast_radius=0.006, skin_thickness=0.0059, src_radius=0.016), p
+
+
+def cylinder_far_pattern(f, phi, src, a, eps_c):
+    """|F(phi)|^2 of a line current at `src` (m, relative to the centre of a
+    cylinder of radius a and complex relative permittivity eps_c, None = no
+    cylinder) in vacuum: F = e^{j k0 d cos(phi - phi_s)} + sum_n a_n j^n e^{jn(phi - phi_s)},
+    a_n as in cylinder_field (the far-field forms of H0^(2)(k0|r - r_s|) and
+    H_n^(2)(k0 rho))."""
+    from scipy.special import h2vp, hankel2, jv, jvp
+    w = 2 * np.pi * f
+    k0 = w * np.sqrt(4e-7 * np.pi * 8.854187817e-12)
+    d, phs = np.hypot(*src), np.arctan2(src[1], src[0])
+    F = np.exp(1j * k0 * d * np.cos(phi - phs)).astype(complex)
+    if eps_c is not None:
+        k1 = k0 * np.sqrt(eps_c)
+        for n in range(-40, 41):
+            cn = (k0 * jvp(n, k0 * a) * jv(n, k1 * a) - k1 * jv(n, k0 * a) * jvp(n, k1 * a)) / \
+                 (k0 * h2vp(n, k0 * a) * jv(n, k1 * a) - k1 * hankel2(n, k0 * a) * jvp(n, k1 * a))
+            F = F + (-hankel2(n, k0 * d) * cn) * (1j ** n) * np.exp(1j * n * (phi - phs))
+    return np.abs(F) ** 2
