@@ -45,12 +45,15 @@
 //       PDEs; physical CPML: reflection against a larger grid; physical running
 //       DFT, closed contour and projection: point-source isotropy and a direct
 //       transform of the samples.
+//   GPU-path Debye current with delta_eps > 0 and the centred fwf origin: a
+//       homogeneous Debye disc lit by the plane wave matches its plane-wave
+//       series (Bessel/Hankel) within 1.5%, as ratios between probes and
+//       absolutely against the incident pulse's spectrum.
 //   Parity unpinned (FDTD_GPU.cu cannot be compiled here and its known-answer
 //   candidate, the commented vector of FDTD_GPU.cu:659-692 against
-//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the GPU
-//   path's Debye current with delta_eps > 0, the centred fwf origin, the phase
-//   output P of N2FPostProcess and the GPU fitness normalisation are restated
-//   from the source only.
+//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the
+//   phase output P of N2FPostProcess and the GPU fitness normalisation are
+//   restated from the source only.
 
 #include <cmath>
 #include <complex>

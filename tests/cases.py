@@ -148,3 +148,67 @@ def cylinder_far_pattern(f, phi, src, a, eps_c):
                  (k0 * h2vp(n, k0 * a) * jv(n, k1 * a) - k1 * hankel2(n, k0 * a) * jvp(n, k1 * a))
             F = F + (-hankel2(n, k0 * d) * cn) * (1j ** n) * np.exp(1j * n * (phi - phs))
     return np.abs(F) ** 2
+
+
+# A homogeneous Debye disc under the FDTD_GPU.cu physics (scattered-field
+# formulation; excitation 0 is a Gaussian plane wave travelling along +x):
+# radius 15.75 mm (31.5 cells of 1 mm), eps_inf 4, delta_eps 10, sigma 0.2 S/m
+# in all nine patches, and a relaxation time of 40 ps, which makes the disc
+# strongly dispersive over 2-3 GHz.  The 601^2 grid puts the CPML far enough
+# out that its echoes reach the probes (30 cells from the centre) only in the
+# last part of the 1300-step record, which a cosine taper removes.
+DISC = (4.0, 10.0, 0.2)  # eps_inf, delta_eps, sigma (S/m)
+DISC_TAU = 40e-12
+DISC_PROBES = [(30, 0), (21, 21), (0, 30), (-21, 21), (-30, 0), (-21, -21), (0, -30), (21, -21), (26, 15),
+               (-28, -10)]
+C0 = 1.0 / np.sqrt(4e-7 * np.pi * 8.854187817e-12)
+
+
+def ref_debye_disc(steps=1300, n=601):
+    c = ref_small(steps=steps, n=n).copy(breast_radius=0.01575, debye_tau=DISC_TAU)
+    p = np.concatenate([np.full(9, DISC[0]), np.full(9, DISC[1]), np.full(9, DISC[2])])
+    return c, p
+
+
+def debye_eps(f, eps_inf, d_eps, sigma, tau):
+    """Complex relative permittivity eps_inf + d_eps / (1 + j w tau) - j sigma / (w eps0)."""
+    w = 2 * np.pi * f
+    return eps_inf + d_eps / (1 + 1j * w * tau) - 1j * sigma / (w * 8.854187817e-12)
+
+
+def plane_wave_scattered(f, pos, a, eps_c, n_terms=40):
+    """Ez scattered at `pos` (m from the disc centre) by a disc of radius a and
+    complex relative permittivity eps_c in vacuum, for a unit plane wave
+    e^{-j k0 x} (e^{+jwt} phasors, travelling along +x, phase zero at the
+    centre): sum_n (-j)^n (-c_n) H_n^(2)(k0 rho) e^{jn phi}, c_n as in
+    cylinder_field (the plane wave's Jacobi-Anger coefficients (-j)^n J_n)."""
+    from scipy.special import h2vp, hankel2, jv, jvp
+    k0 = 2 * np.pi * f / C0
+    k1 = k0 * np.sqrt(eps_c)
+    rho, ph = np.hypot(*pos), np.arctan2(pos[1], pos[0])
+    S = 0j
+    for m in range(-n_terms, n_terms + 1):
+        cn = (k0 * jvp(m, k0 * a) * jv(m, k1 * a) - k1 * jv(m, k0 * a) * jvp(m, k1 * a)) / \
+             (k0 * h2vp(m, k0 * a) * jv(m, k1 * a) - k1 * hankel2(m, k0 * a) * jvp(m, k1 * a))
+        S = S + (-1j) ** m * (-cn) * hankel2(m, k0 * rho) * np.exp(1j * m * ph)
+    return S
+
+
+def disc_incident_spectrum(cfg, f, origin_cells=0.0):
+    """Spectrum at the disc centre of the incident wave exp(-((t - t0 - (x + R)/c0)/tau)^2)
+    with x measured from the grid centre (origin_cells shifts that origin):
+    tau sqrt(pi) e^{-(w tau / 2)^2} e^{-jw (t0 + (R + origin_cells dx)/c0)}."""
+    w = 2 * np.pi * f
+    tp = cfg.pulse_tau
+    delay = cfg.pulse_t0 + (cfg.breast_radius + origin_cells * cfg.dx) / C0
+    return tp * np.sqrt(np.pi) * np.exp(-(w * tp) ** 2 / 4) * np.exp(-1j * w * delay)
+
+
+def tapered(tr, frac=0.3):
+    """The record with a cosine taper over its last `frac`."""
+    tr = np.asarray(tr, np.float64)
+    n = len(tr)
+    m = int(n * frac)
+    w = np.ones(n)
+    w[n - m:] = 0.5 * (1 + np.cos(np.pi * np.arange(m) / m))
+    return tr * w[:, None]
