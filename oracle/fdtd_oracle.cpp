@@ -49,11 +49,13 @@
 //       homogeneous Debye disc lit by the plane wave matches its plane-wave
 //       series (Bessel/Hankel) within 1.5%, as ratios between probes and
 //       absolutely against the incident pulse's spectrum.
+//   Phase output P of N2FPostProcess: the same disc's complex far field
+//       sqrt(D) e^{jP} matches its series far field (with the reference's
+//       phase centre, half a cell off the disc centre for odd nx) within 1.5%.
 //   Parity unpinned (FDTD_GPU.cu cannot be compiled here and its known-answer
 //   candidate, the commented vector of FDTD_GPU.cu:659-692 against
-//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the
-//   phase output P of N2FPostProcess and the GPU fitness normalisation are
-//   restated from the source only.
+//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the GPU
+//   fitness normalisation is restated from the source only.
 
 #include <cmath>
 #include <complex>

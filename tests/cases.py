@@ -194,6 +194,21 @@ def plane_wave_scattered(f, pos, a, eps_c, n_terms=40):
     return S
 
 
+def plane_wave_far_field(f, phi, a, eps_c, n_terms=40):
+    """Far-field form of plane_wave_scattered without its sqrt(2 / (pi k0 rho))
+    e^{-j(k0 rho - pi/4)} factor: H_n^(2)(k0 rho) -> j^n, so the pattern is
+    -sum_n c_n e^{jn phi}."""
+    from scipy.special import h2vp, hankel2, jv, jvp
+    k0 = 2 * np.pi * f / C0
+    k1 = k0 * np.sqrt(eps_c)
+    F = np.zeros(np.shape(phi), complex)
+    for m in range(-n_terms, n_terms + 1):
+        cn = (k0 * jvp(m, k0 * a) * jv(m, k1 * a) - k1 * jv(m, k0 * a) * jvp(m, k1 * a)) / \
+             (k0 * h2vp(m, k0 * a) * jv(m, k1 * a) - k1 * hankel2(m, k0 * a) * jvp(m, k1 * a))
+        F = F - cn * np.exp(1j * m * np.asarray(phi))
+    return F
+
+
 def disc_incident_spectrum(cfg, f, origin_cells=0.0):
     """Spectrum at the disc centre of the incident wave exp(-((t - t0 - (x + R)/c0)/tau)^2)
     with x measured from the grid centre (origin_cells shifts that origin):
