@@ -1305,9 +1305,11 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
 // class in TileItem::cls).  One launch per time block with no cross-stream
 // joins: the form for few simulations, where every block's latency is on the
 // critical path.  Classes are listed slowest first so their tiles start first.
-// Workgroups per CU: kFusedWgPerCu (three for the select-form stages, so a
-// 1024^2 grid's tiles start in one wave of workgroups; two for the
-// branch-free T = 8 stages, whose corner body needs the registers).
+// Workgroups per CU: kFusedWgPerCu.  Three for the select-form stages: the
+// corner body alone needs 160 VGPRs, and four or five workgroups per CU
+// (128 or 96 VGPRs) spill and run slower, although at three a 1024^2 grid's
+// last interior tiles wait for a second round of workgroup slots.  Two for the
+// branch-free T = 8 stages, whose corner body needs the registers.
 template <int T>
 __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem* __restrict__ tiles, int ntiles,
                                                 const LaunchItem* __restrict__ items,
@@ -1330,10 +1332,14 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
         else fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
+        // Boundary tiles set the launch's length (their stages carry the CPML
+        // and contour work): their waves issue first on a shared SIMD.
+        if (cls != TILE_INNER) __builtin_amdgcn_s_setprio(3);
         if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
         else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
         else if (cls == TILE_COL) body_col<T, kFusedGroupCol>(tile, it, nst, Sp, ordinal, lc4, ltab);
         else body_inner<T, kFusedGroupInner>(tile, it, nst, Sp, ordinal, lc4);
+        __builtin_amdgcn_s_setprio(0);
     }
 }
 
