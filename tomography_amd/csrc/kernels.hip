@@ -1480,7 +1480,8 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
 // zero.  A workgroup of four waves owns 32 simulations (8 per wave, one
 // 32-row tile each) x kProjAngles angles.  The contour is walked in chunks of
 // kProjChunk points: the workgroup evaluates the chunk's phase tile W once
-// into LDS, then every wave runs v_mfma_f32_32x32x2f32 over the chunk, two
+// into LDS (double-buffered: the next chunk's tile and current values are
+// prepared while this one runs), then every wave runs v_mfma_f32_32x32x2f32 over the chunk, two
 // points at a time, four products per complex step (Re = XrWr - XiWi,
 // Im = XrWi + XiWr) for each 32-angle tile, reusing its current values X
 // across the angle tiles.  Operand layouts of the 32x32x2 f32 product:
@@ -1504,7 +1505,7 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
                                                                   const float2* __restrict__ kinv, int A,
                                                                   float eta0, float* __restrict__ D,
                                                                   float* __restrict__ Pph) {
-    __shared__ float2 wt[kProjChunk][kProjAngles];  // phase tile {cos, sin}
+    __shared__ float2 wt[2][kProjChunk][kProjAngles];  // phase tiles {cos, sin}, double-buffered
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int a0 = blockIdx.x * kProjAngles, f = blockIdx.z;
     constexpr int AT = SPLIT ? 1 : kProjAngleTiles;     // 32-angle tiles per wave
@@ -1535,45 +1536,62 @@ __global__ __launch_bounds__(64 * kProjWaves) void k_project_mfma(const int* __r
     for (int m = 0; m < AT; ++m)
 #pragma unroll
         for (int r = 0; r < 16; ++r) cre[m][r] = cim[m][r] = 0.f;
-    for (int c0 = 0; c0 < P; c0 += kProjChunk) {
-        __syncthreads();
+    // Chunk c's phase tile and current values are prepared while chunk c - 1
+    // runs on the matrix cores: one barrier per chunk, and the current-value
+    // loads stay in flight across the products (the scale `del` is applied
+    // when they are consumed, so nothing waits on them earlier).
+    auto fill = [&](int c0, int b) {
 #pragma unroll
         for (int j = 0; j < kProjChunk / 2; ++j) {
             const int pp = fp + 2 * j, p = c0 + pp;
             float sn = 0.f, cs = 0.f;
             if (p < P && a0 + fa < A) sincosf(k * (px[p] * fc + py[p] * fs), &sn, &cs);
-            wt[pp][fa] = make_float2(cs, sn);
+            wt[b][pp][fa] = make_float2(cs, sn);
         }
-        // this lane's current values for the chunk (two points per step)
-        float xr[kProjChunk / 2], xi[kProjChunk / 2];
+    };
+    float rr[kProjChunk / 2], ri[kProjChunk / 2], sc[kProjChunk / 2];
+    auto load = [&](int c0) {
 #pragma unroll
         for (int j = 0; j < kProjChunk / 2; ++j) {
             const int p = c0 + 2 * j + kh;
-            xr[j] = xi[j] = 0.f;
+            rr[j] = ri[j] = sc[j] = 0.f;
             if (p < P && accX) {
                 const bool ysf = p < L1 || (p >= L2 && p < L3);
                 if (rowk == 0 || (rowk == 1) == ysf) {
-                    const float del = (ysf ? dxf : dyf) * (rowk == 0 ? -1.f : 1.f);
+                    sc[j] = (ysf ? dxf : dyf) * (rowk == 0 ? -1.f : 1.f);
                     const size_t o = (size_t)f * P + p;
-                    float vr, vi;
-                    if (fp64) { const double2 v = ((const double2*)accX)[o]; vr = (float)v.x; vi = (float)v.y; }
-                    else { const float4 v = ((const float4*)accX)[o]; vr = v.x; vi = v.y; }
-                    xr[j] = del * vr;
-                    xi[j] = del * vi;
+                    if (fp64) { const double2 v = ((const double2*)accX)[o]; rr[j] = (float)v.x; ri[j] = (float)v.y; }
+                    else { const float2 v = ((const float2*)accX)[2 * o]; rr[j] = v.x; ri[j] = v.y; }
                 }
             }
         }
-        __syncthreads();
+    };
+    float xr[kProjChunk / 2], xi[kProjChunk / 2];
+    fill(0, 0);
+    load(0);
+#pragma unroll
+    for (int j = 0; j < kProjChunk / 2; ++j) { xr[j] = sc[j] * rr[j]; xi[j] = sc[j] * ri[j]; }
+    for (int c0 = 0, b = 0; c0 < P; c0 += kProjChunk, b ^= 1) {
+        __syncthreads();  // tile b is complete; tile b ^ 1 is no longer read
+        const bool more = c0 + kProjChunk < P;
+        if (more) {
+            load(c0 + kProjChunk);
+            fill(c0 + kProjChunk, b ^ 1);
+        }
 #pragma unroll
         for (int j = 0; j < kProjChunk / 2; ++j) {
 #pragma unroll
             for (int m = 0; m < AT; ++m) {
-                const float2 w = wt[2 * j + kh][32 * (mt0 + m) + (lane & 31)];
+                const float2 w = wt[b][2 * j + kh][32 * (mt0 + m) + (lane & 31)];
                 cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.x, cre[m], 0, 0, 0);
                 cre[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(-xi[j], w.y, cre[m], 0, 0, 0);
                 cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[j], w.y, cim[m], 0, 0, 0);
                 cim[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(xi[j], w.x, cim[m], 0, 0, 0);
             }
+        }
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < kProjChunk / 2; ++j) { xr[j] = sc[j] * rr[j]; xi[j] = sc[j] * ri[j]; }
         }
     }
     const float inv = ki.y;
