@@ -337,19 +337,38 @@ def run_uneven(ctx, n, plain_sample, batch_sched, plain_sched):
            "update_phase_Mcell_steps_per_s": round(st["update"]["cell_steps"] / (st["update"]["ms"] / 1e3) / 1e6, 1)
            if st["update"]["ms"] else None}
     if plain_sample:
+        # The plain path's time on the whole mix, estimated per grid-size bin
+        # (bins a factor sqrt(2) wide): `plain_sample` seeded requests of each
+        # bin are timed, and the bin's plain seconds per cell-step is applied
+        # to all of its requests.  The cost per cell-step depends mainly on
+        # the grid, and the bins keep the heavy tail (cost ~ N^3) represented.
         ctx.set_schedule(**plain_sched)
-        idx = np.random.default_rng(7).choice(n, plain_sample, replace=False)
-        tp, cp = 0.0, 0.0
-        for i in idx:
-            c, v = reqs[i]
-            t1 = time.perf_counter()
-            ctx.eval(c, v)
-            tp += time.perf_counter() - t1
-            cp += cell_steps(c)
-        # plain evals/s on this mix: the sample's mean seconds per eval
-        out["plain_sample"] = {"evals": int(plain_sample), "s": round(tp, 3), "evals_per_s": round(plain_sample / tp, 3),
-                               "Mcell_steps_per_s": round(cp / tp / 1e6, 1)}
-        out["batched_vs_plain_evals_per_s"] = round(out["evals_per_s"] / out["plain_sample"]["evals_per_s"], 3)
+        edges = [256, 362, 512, 724, 1024, 1448, 2049]
+        rng = np.random.default_rng(7)
+        bins, pred, tt, nt = [], 0.0, 0.0, 0
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            members = [i for i, (c, _) in enumerate(reqs) if lo <= c.nx < hi]
+            if not members:
+                continue
+            pick = rng.choice(members, min(plain_sample, len(members)), replace=False)
+            tp, cp = 0.0, 0.0
+            for i in pick:
+                c, v = reqs[i]
+                t1 = time.perf_counter()
+                ctx.eval(c, v)
+                tp += time.perf_counter() - t1
+                cp += cell_steps(c)
+            cb = sum(cell_steps(reqs[i][0]) for i in members)
+            pred += cb * tp / cp
+            tt += tp
+            nt += len(pick)
+            bins.append({"nx": [lo, hi], "requests": len(members), "timed": len(pick),
+                         "plain_Mcell_steps_per_s": round(cp / tp / 1e6, 1)})
+        out["plain_mix_estimate"] = {
+            "method": "per grid-size bin: plain seconds per cell-step of the timed requests x the bin's cell-steps",
+            "requests_timed": nt, "s_timed": round(tt, 3), "predicted_s": round(pred, 3),
+            "evals_per_s": round(n / pred, 3), "Mcell_steps_per_s": round(cells / pred / 1e6, 1), "bins": bins}
+        out["batched_vs_plain_evals_per_s"] = round(out["evals_per_s"] / out["plain_mix_estimate"]["evals_per_s"], 3)
     return out
 
 
@@ -529,7 +548,8 @@ def main():
                     help="plain-path evals on the batched workload (for the batched/plain ratio; 0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
     ap.add_argument("--uneven", type=int, default=0, help="requests of the uneven-batch study (512 in its definition)")
-    ap.add_argument("--uneven-plain-sample", type=int, default=8, help="plain-path evals timed on the uneven mix")
+    ap.add_argument("--uneven-plain-sample", type=int, default=2,
+                    help="plain-path evals timed per grid-size bin of the uneven mix")
     ap.add_argument("--precision", type=int, default=0, help="results of the FP32-Kahan vs FP64 precision study")
     args = ap.parse_args()
     if args.gpus != 1:
