@@ -118,13 +118,18 @@ def kernel_ms(st, per=1.0):
     return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
 
 
-SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form", "edge_rows")
-# Default schedules (picked by Context.autotune on MI355X; fixed so that the
-# profiled command and the timed one run the same launches).
-PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=6, edge_waves=4, inner_waves=4, chunk=64, grid_mult=16, form=2, edge_rows=2)
+SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form", "edge_rows",
+              "inner_rows")
+# Default schedules (swept on MI355X; fixed so that the profiled command and
+# the timed one run the same launches).  The plain one: fused T = 4 blocks,
+# one wave per workgroup, 1-row corner / row-edge tiles, 4-row column-edge
+# tiles and 8-row interior tiles.
+PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=4, edge_waves=1, inner_waves=1, chunk=64, grid_mult=16, form=2, edge_rows=1,
+                      inner_rows=8)
 BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=64, grid_mult=16, form=1,
-                      edge_rows=8)
-REF_SCHEDULE = dict(tsteps=2, seg_rows=8, edge_waves=2, inner_waves=2, chunk=32, grid_mult=8, form=2, edge_rows=4)
+                      edge_rows=8, inner_rows=0)
+REF_SCHEDULE = dict(tsteps=2, seg_rows=8, edge_waves=2, inner_waves=2, chunk=32, grid_mult=8, form=2, edge_rows=4,
+                    inner_rows=0)
 
 
 def apply_schedule(ctx, cfg, n_sims, sched, tune):

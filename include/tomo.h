@@ -141,13 +141,16 @@ const char *tomo_last_error(void);
 #define TOMO_FORM_FUSED 2
 typedef struct tomo_schedule {
     int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows;
+    /* Output rows of an interior tile of a physical-mode simulation; 0 = seg_rows
+     * (which then sets the column-edge and the interior tiles alike). */
+    int32_t inner_rows;
 } tomo_schedule;
 int tomo_set_schedule(tomo_ctx *ctx, const tomo_schedule *s);
 int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
 
 /* Autotuner: time candidate schedules on n_sims resident simulations of
- * `cfg` (coordinate descent over form, tsteps, seg_rows, edge_rows,
- * edge/inner waves and grid_mult; the chunk is kept), install the fastest and return it with its
+ * `cfg` (coordinate descent over form, tsteps, seg_rows, inner_rows,
+ * edge_rows, edge/inner waves and grid_mult; the chunk is kept), install the fastest and return it with its
  * measured update rate in cell-steps per second. */
 int tomo_autotune(tomo_ctx *ctx, const tomo_sim_config *cfg, int n_sims, tomo_schedule *best,
                   double *cell_steps_per_s);

@@ -34,7 +34,7 @@ class Schedule(_c.Structure):
 
     _fields_ = [("tsteps", _c.c_int32), ("seg_rows", _c.c_int32), ("edge_waves", _c.c_int32),
                 ("inner_waves", _c.c_int32), ("chunk", _c.c_int32), ("grid_mult", _c.c_int32),
-                ("form", _c.c_int32), ("edge_rows", _c.c_int32)]
+                ("form", _c.c_int32), ("edge_rows", _c.c_int32), ("inner_rows", _c.c_int32)]
 
     def as_tuple(self):
         return tuple(getattr(self, k) for k, _ in self._fields_)
@@ -179,8 +179,8 @@ class Context:
         self.close()
 
     def set_schedule(self, tsteps=0, seg_rows=0, edge_waves=0, chunk=0, inner_waves=0, grid_mult=0, form=0,
-                     edge_rows=0):
-        s = Schedule(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows)
+                     edge_rows=0, inner_rows=0):
+        s = Schedule(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows, inner_rows)
         _check(lib().tomo_set_schedule(self.h, _c.byref(s)))
 
     def autotune(self, cfg, n_sims):
@@ -223,7 +223,7 @@ class Context:
         return best, res.as_dict()
 
     def schedule(self):
-        """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows)"""
+        """(tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows, inner_rows)"""
         s = Schedule()
         _check(lib().tomo_get_schedule(self.h, _c.byref(s)))
         return s.as_tuple()

@@ -102,6 +102,7 @@ struct tomo_ctx {
     int grid_mult = 4;                 // persistent workgroups per CU
     int form = TOMO_FORM_STREAMS;      // launch form of physical-mode blocks
     int ER = 8;                        // output rows per corner / row-edge tile
+    int Si = 0;                        // output rows per interior tile (0: S)
     bool use_graph = false;            // replay regular epochs from a captured hipGraph
     hipGraphExec_t gexec = nullptr;    // the captured epoch and what it was captured for
     std::vector<uint64_t> gkey_serials;
@@ -120,7 +121,7 @@ struct tomo_ctx {
     hipEvent_t staged = nullptr;
     std::vector<uint64_t> tile_key;  // serials of the simulations the uploaded tile list was built for
     uint64_t next_serial = 1;
-    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0, tile_key_ER = 0;
+    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0, tile_key_ER = 0, tile_key_Si = 0;
     int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
     int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
     int n_corner = 0;
@@ -377,7 +378,7 @@ int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); 
 // full-width bands of the clean interior go to the interior kernel (physical
 // mode only).  Interior bands start exactly where the clean zone starts, so
 // the edge frame is one narrow band wide on each side.
-void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, int ER, std::vector<TileItem>& edge,
+void build_tiles(Slot* sl, int item, int T, int S, int Si, int W, int Wi, int ER, std::vector<TileItem>& edge,
                  std::vector<TileItem>& rowedge, std::vector<TileItem>& coledge, std::vector<TileItem>& inner) {
     const DevSim& d = sl->d;
     for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
@@ -439,12 +440,14 @@ void build_tiles(Slot* sl, int item, int T, int S, int W, int Wi, int ER, std::v
         for (int a = y0; a < y1; a += sc) add(1, rowedge, ixl, ixr, a, std::min(a + sc, y1), Wi);
     };
     edge_rows_split(0, ia);
+    // column-edge tiles S rows tall, interior tiles Si (their stages are
+    // shorter, so taller interior tiles finish with the column-edge ones)
     for (int a = ia; a < ib; a += S) {
         const int a1 = std::min(a + S, ib);
         add(2, coledge, 0, ixl, a, a1, W);
-        add(3, inner, ixl, ixr, a, a1, Wi);
         add(2, coledge, ixr, nx + 1, a, a1, W);
     }
+    for (int a = ia; a < ib; a += Si) add(3, inner, ixl, ixr, a, std::min(a + Si, ib), Wi);
     edge_rows_split(ib, ny + 1);
 }
 
@@ -526,10 +529,11 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             for (Slot* s : ms) key.push_back(s->serial);
             std::vector<TileItem> te, tr, tc, tn;
             const bool rebuild = key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S ||
-                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi || c->tile_key_ER != c->ER;
+                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi || c->tile_key_ER != c->ER ||
+                                 c->tile_key_Si != c->Si;
             if (rebuild) {
                 for (size_t i = 0; i < ms.size(); ++i)
-                    build_tiles(ms[i], (int)i, T, S, W, c->Wi, c->ER, te, tr, tc, tn);
+                    build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
             }
             const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : c->ntiles;
             // Items of E consecutive epochs in one upload: every simulation of
@@ -568,6 +572,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 c->tile_key = key;
                 c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W; c->tile_key_Wi = c->Wi;
                 c->tile_key_ER = c->ER;
+                c->tile_key_Si = c->Si;
                 c->ntiles = ntiles;
                 c->n_edge = (int)(te.size() + tr.size() + tc.size());
                 c->n_corner = (int)te.size();
@@ -683,7 +688,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 }
                 nl = (maxn + T - 1) / T;
                 if (c->use_graph && !c->prof && minn == K) {
-                    const std::vector<int64_t> gk = {mode, T, S, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
+                    const std::vector<int64_t> gk = {mode, T, S, c->Si, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
                                                      fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)d_it,
                                                      (int64_t)(intptr_t)c->d_tiles, (int64_t)(intptr_t)c->d_sims};
                     if (!c->gexec || gk != c->gkey || key != c->gkey_serials) {
@@ -1156,9 +1161,11 @@ static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
     const int ER = s.edge_rows ? s.edge_rows : c->ER;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
     if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64 ||
-        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED) || ER < 1 || ER > 4096)
+        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED) || ER < 1 || ER > 4096 ||
+        (s.inner_rows != 0 && (s.inner_rows < 4 || s.inner_rows > 4096)))
         return fail(TOMO_E_INVALID, "bad schedule");
     c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F; c->ER = ER;
+    c->Si = s.inner_rows;
     return TOMO_OK;
 }
 
@@ -1175,6 +1182,7 @@ int tomo_get_schedule(tomo_ctx* c, tomo_schedule* s) {
     if (!c || !s) return fail(TOMO_E_INVALID, "null argument");
     s->tsteps = c->T; s->seg_rows = c->S; s->edge_waves = c->W; s->inner_waves = c->Wi;
     s->chunk = c->K; s->grid_mult = c->grid_mult; s->form = c->form; s->edge_rows = c->ER;
+    s->inner_rows = c->Si;
     return TOMO_OK;
 }
 
@@ -1224,7 +1232,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     }
     const bool prof = c->prof;
     c->prof = false;
-    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form, c->ER};
+    tomo_schedule cur{c->T, c->S, c->W, c->Wi, c->K, c->grid_mult, c->form, c->ER, c->Si};
     // each trial times at least 512 steps: a few launches alone are dominated
     // by first-launch and dispatch effects
     const int epochs = std::max(2, (512 + c->K - 1) / c->K);
@@ -1245,19 +1253,20 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     // poor corner of the space: one simulation (fused, short tiles) and many
     // (four streams, tall tiles).
     if (tc.mode == TOMO_MODE_PHYSICAL) {
-        const tomo_schedule seeds[2] = {{4, 6, 4, 4, c->K, 16, TOMO_FORM_FUSED, 2},
-                                        {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 8}};
+        const tomo_schedule seeds[2] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 8},
+                                        {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 8, 0}};
         for (const tomo_schedule& sd : seeds)
             if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
     }
     const int Ts[] = {1, 2, 4, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
-    const int ERs[] = {1, 2, 3, 4, 8, 16};
+    const int ERs[] = {1, 2, 3, 4, 8, 16}, SIs[] = {0, 6, 8, 12, 16};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
             for (int v : Fs) { tomo_schedule x = cur; x.form = v; if (!rc) rc = trial(x); }
         for (int v : Ts) { tomo_schedule x = cur; x.tsteps = v; if (!rc && c->K % v == 0) rc = trial(x); }
         for (int v : Ss) { tomo_schedule x = cur; x.seg_rows = v; if (!rc) rc = trial(x); }
+        for (int v : SIs) { tomo_schedule x = cur; x.inner_rows = v; if (!rc) rc = trial(x); }
         for (int v : ERs) { tomo_schedule x = cur; x.edge_rows = v; if (!rc) rc = trial(x); }
         for (int v : Ws) { tomo_schedule x = cur; x.edge_waves = v; if (!rc) rc = trial(x); }
         for (int v : Ws) { tomo_schedule x = cur; x.inner_waves = v; if (!rc) rc = trial(x); }
