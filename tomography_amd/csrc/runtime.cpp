@@ -1253,13 +1253,13 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     // poor corner of the space: one simulation (fused, short tiles) and many
     // (four streams, tall tiles).
     if (tc.mode == TOMO_MODE_PHYSICAL) {
-        const tomo_schedule seeds[2] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 8},
+        const tomo_schedule seeds[2] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 14},
                                         {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 8, 0}};
         for (const tomo_schedule& sd : seeds)
             if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
     }
     const int Ts[] = {1, 2, 4, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
-    const int ERs[] = {1, 2, 3, 4, 8, 16}, SIs[] = {0, 6, 8, 12, 16};
+    const int ERs[] = {1, 2, 3, 4, 8, 16}, SIs[] = {0, 8, 12, 14, 16, 24};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
