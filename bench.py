@@ -36,6 +36,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import tomography_amd as t  # noqa: E402
+from tomography_amd.workloads import (BATCH_SCHEDULE, PLAIN_SCHEDULE, REF_SCHEDULE, SCHED_KEYS, SERVE_ANTENNAS,  # noqa: E402
+                                      SERVE_GRIDS, batch_vectors, cell_steps, seeded_vectors, serve_requests,
+                                      uneven_requests)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-steps/s & fitness evals/s, batched vs plain; p50/p99 eval latency"
@@ -44,23 +47,6 @@ METRIC = "Mcell-steps/s & fitness evals/s, batched vs plain; p50/p99 eval latenc
 def log(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def seeded_vectors(n, dim, seed):
-    """Parameter vectors inside the physical bounds (tumour centre, radius, Debye values)."""
-    rng = np.random.default_rng(seed)
-    v = np.zeros((n, dim))
-    for b in range(0, dim, 8):
-        k = min(8, dim - b)
-        blk = np.stack([rng.uniform(-15, 15, n), rng.uniform(-15, 15, n), rng.uniform(2, 6, n),
-                        rng.uniform(10, 30, n), rng.uniform(10, 50, n), rng.uniform(8, 20, n),
-                        rng.uniform(0, 1, n), rng.uniform(0, 4, n)], axis=1)
-        v[:, b:b + k] = blk[:, :k]
-    return v
-
-
-def cell_steps(cfg):
-    return float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
 
 
 UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")
@@ -118,20 +104,6 @@ def kernel_ms(st, per=1.0):
     return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
 
 
-SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult", "form", "edge_rows",
-              "inner_rows")
-# Default schedules (swept on MI355X; fixed so that the profiled command and
-# the timed one run the same launches).  The plain one: fused T = 4 blocks,
-# one wave per workgroup, 1-row corner / row-edge tiles, 4-row column-edge
-# tiles and 14-row interior tiles.
-PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=4, edge_waves=1, inner_waves=1, chunk=64, grid_mult=16, form=2, edge_rows=1,
-                      inner_rows=14)
-BATCH_SCHEDULE = dict(tsteps=4, seg_rows=32, edge_waves=1, inner_waves=4, chunk=64, grid_mult=16, form=1,
-                      edge_rows=8, inner_rows=0)
-REF_SCHEDULE = dict(tsteps=2, seg_rows=8, edge_waves=2, inner_waves=2, chunk=32, grid_mult=8, form=2, edge_rows=4,
-                    inner_rows=0)
-
-
 def apply_schedule(ctx, cfg, n_sims, sched, tune):
     if tune:
         ctx.autotune(cfg, n_sims)
@@ -163,10 +135,6 @@ def run_plain(ctx, steps, warmup, sched, tune):
     return cfg, dt, st, used, dt_prof
 
 
-def batch_vectors():
-    return seeded_vectors(64, 8, 20261203)
-
-
 def run_batched(ctx, sched, tune):
     """cfg3: 64 vectors x 16 antennas = 1024 simulations advanced together."""
     cfg = t.preset(t.PRESET_CFG3)
@@ -196,23 +164,6 @@ def run_plain_same_workload(ctx, n, sched, batch_fits):
     rel = max(abs(f - b) / abs(b) for f, b in zip(fits, batch_fits[:n]))
     return {"evals": n, "s": round(dt, 3), "evals_per_s": round(n / dt, 3),
             "Mcell_steps_per_s": round(cell_steps(cfg) * n / dt / 1e6, 1), "max_rel_fitness_diff_vs_batched": rel}
-
-
-SERVE_GRIDS = (256, 362, 512, 724, 1024, 1448, 2048)  # log-spaced 256^2..2048^2
-SERVE_ANTENNAS = (1, 2, 4, 8, 16)
-
-
-def serve_requests(n_req, seed=20261205):
-    """Serving mix: grids log-spaced 256^2..2048^2, 1-16 antennas, 8-32-dim
-    vectors (1-4 tumour blocks), drawn per request."""
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(n_req):
-        g = int(rng.choice(SERVE_GRIDS))
-        cfg = t.preset(t.PRESET_CFG5, g).copy(n_exc=int(rng.choice(SERVE_ANTENNAS)))
-        dim = int(rng.choice([8, 16, 24, 32]))
-        reqs.append((cfg, seeded_vectors(1, dim, seed + 1 + i)[0]))
-    return reqs
 
 
 def run_serve(ctx, n_req, rates, plain_sample, batch_sched, plain_sched, seed=20261205):
@@ -288,23 +239,6 @@ def run_search(ctx, method, pop, evals, inflight, sched):
                 "Mcell_steps_per_s": round(res["evals"] * cell_steps(cfg) / res["wall_s"] / 1e6, 1),
                 "best_x": [round(float(x), 3) for x in best]})
     return res
-
-
-def uneven_requests(n, seed=20261204):
-    """The uneven batch: grids log-uniform in [256, 2048] (steps clamp(8N, 2048,
-    16384), one antenna each) and 0-3 tumours per phantom (0: a zero-radius
-    tumour block)."""
-    rng = np.random.default_rng(seed)
-    grids = np.exp(rng.uniform(np.log(256), np.log(2048), n)).astype(int)
-    ntum = rng.integers(0, 4, n)
-    reqs = []
-    for i in range(n):
-        cfg = t.preset(t.PRESET_CFG4, int(grids[i]))
-        v = seeded_vectors(1, 8 * max(1, int(ntum[i])), seed + 1 + i)[0]
-        if ntum[i] == 0:
-            v[2] = 0.0
-        reqs.append((cfg, v))
-    return reqs
 
 
 def run_uneven(ctx, n, plain_sample, batch_sched, plain_sched):

@@ -124,6 +124,11 @@ typedef struct tomo_ctx tomo_ctx;
 /* Create a context on HIP device `device` with a device-memory budget for
  * resident simulations (bytes; 0 = 90% of free memory). */
 int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx **out);
+/* Destroy a context: frees every resident simulation it holds (requests in
+ * any state, and the slots of tomo_sim handles).  A tomo_sim handle outlives
+ * its context safely: afterwards tomo_sim_destroy only frees the handle and
+ * every other tomo_sim_* call returns TOMO_E_STATE.  Must not run
+ * concurrently with another call on the same context or its handles. */
 int tomo_ctx_destroy(tomo_ctx *ctx);
 const char *tomo_last_error(void);
 
@@ -183,6 +188,9 @@ typedef struct tomo_kernel_stats {
 typedef struct tomo_stats {
     tomo_kernel_stats kernel[TOMO_KERNEL_COUNT];
     tomo_kernel_stats update; /* launches = epochs */
+    /* request records the context holds: queued, resident, or finished or
+     * cancelled and not yet collected by tomo_poll / tomo_wait */
+    int64_t requests;
 } tomo_stats;
 int tomo_profile(tomo_ctx *ctx, int enable); /* (re)starts the counters */
 /*

@@ -45,6 +45,8 @@ class Oracle:
         L.orc_source_cell.argtypes = [_c.c_void_p, _c.c_int, _ip, _ip]
         L.orc_incident_power.argtypes = [_c.c_void_p, _c.c_double]
         L.orc_incident_power.restype = _c.c_double
+        L.orc_farfield.argtypes = [_c.c_void_p, _dp, _c.c_size_t, _c.c_int, _c.c_int, _dp, _dp]
+        L.orc_fitness.argtypes = [_c.c_void_p, _dp, _dp, _fp, _fp, _c.c_size_t, _dp]
 
     def err(self):
         return self.lib.orc_last_error().decode()
@@ -70,6 +72,29 @@ class Oracle:
         if rc != 0:
             raise ValueError(self.err())
         return fit.value, D, P
+
+    def farfield(self, cfg, params, exc, fp64=True):
+        """Far field (D, P), [freq][angle], of one excitation run to cfg.steps.
+        Releases the GIL for the whole run, so host threads run several at once."""
+        p = np.ascontiguousarray(params, np.float64)
+        D = np.zeros(cfg.n_freq * cfg.n_angles)
+        P = np.zeros_like(D)
+        if self.lib.orc_farfield(_c.byref(cfg), _ptr(p, _dp), p.size, int(exc), int(fp64), _ptr(D, _dp),
+                                 _ptr(P, _dp)) != 0:
+            raise ValueError(self.err())
+        return D.reshape(cfg.n_freq, cfg.n_angles), P.reshape(cfg.n_freq, cfg.n_angles)
+
+    def fitness(self, cfg, D, target, P=None, target_phase=None):
+        """The oracle's fitness arithmetic on given far-field records (cfg.flags select the form)."""
+        D = np.ascontiguousarray(D, np.float64).ravel()
+        t = np.ascontiguousarray(target, np.float32).ravel()
+        Pp = None if P is None else np.ascontiguousarray(P, np.float64).ravel()
+        tp = None if target_phase is None else np.ascontiguousarray(target_phase, np.float32).ravel()
+        assert t.size >= D.size and (Pp is None or Pp.size == D.size) and (tp is None or tp.size >= D.size)
+        f = _c.c_double()
+        self.lib.orc_fitness(_c.byref(cfg), _ptr(D, _dp), _ptr(Pp, _dp), _ptr(t, _fp), _ptr(tp, _fp), D.size,
+                             _c.byref(f))
+        return f.value
 
     def target_vector(self, cfg):
         v = np.zeros(8)
@@ -158,6 +183,8 @@ class Reference:
         L.ref_contour.argtypes = [_ip, _ip]
         L.ref_incident_power.argtypes = [_c.c_float]
         L.ref_incident_power.restype = _c.c_float
+        L.ref_fitness.argtypes = [_fp, _c.c_int, _fp]
+        L.ref_fitness.restype = _c.c_float
 
     @staticmethod
     def available(path=_REF):
@@ -187,6 +214,13 @@ class Reference:
                          _ptr(tr, _fp), _ptr(cj, _fp), _ptr(D, _fp))
         return dict(ez=ez.reshape(402, 402), hx=hx.reshape(401, 401), hy=hy.reshape(401, 401),
                     trace=tr[:nsteps], cj=cj, D=None if D is None else D.reshape(10, 100))
+
+    def fitness(self, D, measurement):
+        """fitness() of FDTD_common.cxx:53-61, compiled from the reference."""
+        D = np.ascontiguousarray(D, np.float32).ravel()
+        m = np.ascontiguousarray(measurement, np.float32).ravel()
+        assert m.size >= D.size
+        return float(self.lib.ref_fitness(_ptr(D, _fp), D.size, _ptr(m, _fp)))
 
     def contour(self):
         xs = np.zeros(800, np.int32)

@@ -24,7 +24,7 @@
 //   calculate_JandM           FDTD_GPU.cu:231-280,   FDTD_CPU.cxx:146-197
 //   calc_incident_power       FDTD_common.cxx:48-51
 //   N2FPostProcess            FDTD_GPU.cu:553-634,   FDTD_CPU.cxx:538-614
-//   fitness                   FDTD_GPU.cu:1004-1008, FDTD_CPU.cxx:841-845,
+//   fitness                   FDTD_GPU.cu:1004-1018, FDTD_CPU.cxx:841-845,
 //                             FDTD_common.cxx:53-61 (TOMO_FLAG_FITNESS_PER_EXC)
 // Deviation from FDTD_CPU.cxx: the far-field loop at FDTD_CPU.cxx:816 reads an
 // uninitialised freq_index (so D stays zero there); the restatement starts
@@ -854,6 +854,40 @@ struct Handle {
     Sim<double> sd;
 };
 
+// Fitness of far-field records D, P (n_exc x n_freq x n_angles each) against
+// a target D (and, for TOMO_FLAG_PHASE_FITNESS, a target phase), in FP64:
+//   FDTD_GPU.cu:1004-1008   -sum (D - m)^2 / (n_exc n_freq)
+//   FDTD_CPU.cxx:841-845    -sum (D - m)^2 / (m^2 n_exc n_freq)   (TOMO_FLAG_REL_FITNESS)
+//   FDTD_common.cxx:53-61   -sum (m - D)^2 / (m^2 n_exc)          (+ TOMO_FLAG_FITNESS_PER_EXC)
+//   FDTD_GPU.cu:1009-1018   phase term -dphi^2 / (10 pi^2 n_exc n_freq), dphi wrapped to [-pi, pi]
+static double fitness_of(const tomo_sim_config& c, const double* D, const double* P, const double* tgt,
+                         const float* tphase, size_t L) {
+    double fit = 0;
+    const double norm = (c.flags & TOMO_FLAG_FITNESS_PER_EXC) ? (double)c.n_exc : (double)c.n_exc * c.n_freq;
+    const bool rel = (c.flags & TOMO_FLAG_REL_FITNESS) != 0;
+    for (size_t i = 0; i < L; ++i) fit -= (D[i] - tgt[i]) * (D[i] - tgt[i]) / (rel ? tgt[i] * tgt[i] * norm : norm);
+    if ((c.flags & TOMO_FLAG_PHASE_FITNESS) && tphase && P) {
+        for (size_t i = 0; i < L; ++i) {
+            double dphi = P[i] - (double)tphase[i];
+            if (dphi > kPI) dphi -= 2 * kPI;
+            else if (dphi < -kPI) dphi += 2 * kPI;
+            fit -= dphi * dphi / (10 * kPI * kPI * norm);
+        }
+    }
+    return fit;
+}
+
+// Far field of one excitation run for c.steps steps.
+template <class R>
+static bool farfield_one(const tomo_sim_config& c, const double* p, size_t np, int exc, std::vector<double>& D,
+                         std::vector<double>& P) {
+    Sim<R> s;
+    if (!sim_init(s, c, p, np, exc)) return false;
+    sim_step(s, c.steps);
+    sim_farfield(s, D, P);
+    return true;
+}
+
 // Full evaluation: every excitation, far field, fitness against target.
 template <class R>
 static bool eval_full(const tomo_sim_config& c, const double* p, size_t np, const float* target, size_t nt,
@@ -885,22 +919,8 @@ static bool eval_full(const tomo_sim_config& c, const double* p, size_t np, cons
         if (target == nullptr || nt < L) { g_err = "target too short"; return false; }
         for (size_t i = 0; i < L; ++i) tgt[i] = target[i];
     }
-    double fit = 0;
-    // FDTD_GPU.cu:1008 and FDTD_CPU.cxx:844 divide by n_exc * n_freq; the generic
-    // fitness() of FDTD_common.cxx:53-61 by n_exc alone
-    const double norm = (c.flags & TOMO_FLAG_FITNESS_PER_EXC) ? (double)c.n_exc : (double)c.n_exc * F;
-    const bool rel = (c.flags & TOMO_FLAG_REL_FITNESS) != 0;  // FDTD_CPU.cxx:844
-    for (size_t i = 0; i < L; ++i)
-        fit -= (Dall[i] - tgt[i]) * (Dall[i] - tgt[i]) / (rel ? tgt[i] * tgt[i] * norm : norm);
-    if ((c.flags & TOMO_FLAG_PHASE_FITNESS) && target != nullptr && nt >= 2 * L) {
-        for (size_t i = 0; i < L; ++i) {
-            double dphi = Pall[i] - (double)target[L + i];
-            if (dphi > kPI) dphi -= 2 * kPI;
-            else if (dphi < -kPI) dphi += 2 * kPI;
-            fit -= dphi * dphi / (10 * kPI * kPI * norm);
-        }
-    }
-    *fitness = fit;
+    const float* tphase = (target != nullptr && nt >= 2 * L) ? target + L : nullptr;
+    *fitness = fitness_of(c, Dall.data(), Pall.data(), tgt.data(), tphase, L);
     if (Dout) std::copy(Dall.begin(), Dall.end(), Dout);
     if (Pout) std::copy(Pall.begin(), Pall.end(), Pout);
     return true;
@@ -1013,8 +1033,27 @@ int orc_sim_farfield(void* hp, double* D, double* P) {
     return 0;
 }
 
-// Fitness of an arbitrary set of DFT accumulators is not needed; the full
-// evaluation below mirrors FDTD_GPU(): every excitation, far field, fitness.
+// Far field D, P ([freq][angle], FP64 results of the R-precision run) of one
+// excitation run to c->steps: the unit a test can spread over host threads.
+int orc_farfield(const tomo_sim_config* c, const double* p, size_t np, int exc, int fp64, double* D, double* P) {
+    std::vector<double> d, pp;
+    const bool ok = fp64 ? farfield_one<double>(*c, p, np, exc, d, pp) : farfield_one<float>(*c, p, np, exc, d, pp);
+    if (!ok) return -1;
+    std::copy(d.begin(), d.end(), D);
+    if (P) std::copy(pp.begin(), pp.end(), P);
+    return 0;
+}
+
+// Fitness of given far-field records (the arithmetic orc_eval applies);
+// tphase may be NULL, P too unless TOMO_FLAG_PHASE_FITNESS is set.
+int orc_fitness(const tomo_sim_config* c, const double* D, const double* P, const float* target,
+                const float* tphase, size_t L, double* fitness) {
+    std::vector<double> tgt(target, target + L);
+    *fitness = fitness_of(*c, D, P, tgt.data(), tphase, L);
+    return 0;
+}
+
+// The full evaluation mirrors FDTD_GPU(): every excitation, far field, fitness.
 int orc_eval(const tomo_sim_config* c, const double* p, size_t np, const float* target, size_t nt, int fp64,
              double* fitness, double* D, double* P) {
     bool ok = fp64 ? eval_full<double>(*c, p, np, target, nt, fitness, D, P)

@@ -49,6 +49,7 @@ void cpu_scattered_parameter_init(float*, float*, float*, float*);
 int CPUgetxfromthreadIdNF2FF(int);
 int CPUgetyfromthreadIdNF2FF(int);
 float calc_incident_power(float);
+float fitness(float*, int, float*);  // FDTD_common.cxx:53-61
 
 // Geometry of the compiled reference (FDTD_common.hxx:24-41 at dx = 1 mm).
 static const int NX = 401, NY = 401, NC = 10, DNF = 100, NFREQ = 10, NOBS = 100;
@@ -70,6 +71,9 @@ int ref_contour(int* xs, int* ys) {
 }
 
 float ref_incident_power(float f) { return calc_incident_power(f); }
+
+// The reference's generic fitness (FDTD_common.cxx:53-61) on caller arrays.
+float ref_fitness(float* D, int n, float* measurement) { return fitness(D, n, measurement); }
 
 // CPML tables as FDTD_CPU() builds them (FDTD_CPU.cxx:765-776): 6 x 10 floats
 // in the order kex, kmx, aex, bex, amx, bmx.

@@ -4,9 +4,10 @@ The reference compiles its target far field into the source
 (FDTD_GPU.cu:996, 8000 floats = D[4][10][100] then the phase P in the same
 layout; FDTD_CPU.cxx:789, 4000 floats = D).  The drop-in shim reads them from
 measurement_gpu.f32 / measurement_cpu.f32 (raw little-endian float32) so that
-the target is data, not code.  Run once in a tree that has the reference:
+the target is data, not code.  It writes into tomography_amd/data/, the
+package data the shim loads.  Run once in a tree that has the reference:
 
-    python tomography_amd/data/extract_measurements.py /root/reference
+    python tests/golden/extract_measurements.py /root/reference
 """
 import os
 import re
@@ -15,6 +16,7 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(os.path.dirname(os.path.dirname(HERE)), "tomography_amd", "data")
 
 
 def extract(path, n):
@@ -29,8 +31,8 @@ def extract(path, n):
 
 
 def main(ref):
-    extract(os.path.join(ref, "FDTD_GPU.cu"), 8000).tofile(os.path.join(HERE, "measurement_gpu.f32"))
-    extract(os.path.join(ref, "FDTD_CPU.cxx"), 4000).tofile(os.path.join(HERE, "measurement_cpu.f32"))
+    extract(os.path.join(ref, "FDTD_GPU.cu"), 8000).tofile(os.path.join(DATA, "measurement_gpu.f32"))
+    extract(os.path.join(ref, "FDTD_CPU.cxx"), 4000).tofile(os.path.join(DATA, "measurement_cpu.f32"))
 
 
 if __name__ == "__main__":

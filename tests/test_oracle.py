@@ -298,21 +298,101 @@ def test_physical_cpml_reflection_is_small(oracle):
     assert _cpml_reflection(oracle, sigma_max=0.0) > 1e-2
 
 
-def test_generic_fitness_normalisation_matches_fdtd_common(oracle):
-    """fitness() of FDTD_common.cxx:53-61 is -sum (m - D)^2 / (n_exc m^2): with
-    TOMO_FLAG_REL_FITNESS | TOMO_FLAG_FITNESS_PER_EXC the oracle's fitness is
-    the relative form times n_freq (it drops the per-frequency average)."""
-    cfg = ref_small(steps=120).copy(n_exc=2)
-    p = ref_params(4)
-    rng = np.random.default_rng(1)
-    target = rng.uniform(0.01, 0.05, cfg.farfield_len()).astype(np.float32)
-    rel = cfg.copy(flags=cfg.flags | t.FLAG_REL_FITNESS)
-    com = cfg.copy(flags=cfg.flags | t.FLAG_REL_FITNESS | t.FLAG_FITNESS_PER_EXC)
-    f_rel, D, _ = oracle.eval(rel, p, target=target, fp64=True)
-    f_com, _, _ = oracle.eval(com, p, target=target, fp64=True)
-    direct = -float(np.sum((target.astype(np.float64) - D) ** 2 / (cfg.n_exc * target.astype(np.float64) ** 2)))
-    assert abs(f_com - direct) <= 1e-12 * abs(direct)
-    assert abs(f_com - f_rel * cfg.n_freq) <= 1e-12 * abs(f_com)
+def _ref_gpu_cfg(flags):
+    c = t.preset(t.PRESET_REF_GPU)
+    return c.copy(flags=c.flags | flags)
+
+
+def _fitness_case(seed, lo=0.01, hi=0.05):
+    rng = np.random.default_rng(seed)
+    L = t.preset(t.PRESET_REF_GPU).farfield_len()
+    return rng.uniform(lo, hi, L).astype(np.float32), rng.uniform(lo, hi, L).astype(np.float32)
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (no reference tree)")
+def test_generic_fitness_matches_compiled_fdtd_common_fitness(oracle):
+    """fitness() of FDTD_common.cxx:53-61, compiled from the reference, on
+    random D and measurement arrays: the oracle with TOMO_FLAG_REL_FITNESS |
+    TOMO_FLAG_FITNESS_PER_EXC agrees within the reference's float accumulation
+    (4000 same-sign terms: relative bound 4000 * 2^-24 = 2.4e-4, observed far
+    below).  Planted mistakes: the per-frequency average (REL alone) or no
+    division by m^2 (PER_EXC alone) miss by far more."""
+    ref = Reference()
+    for seed in (1, 2, 3):
+        D, m = _fitness_case(seed)
+        want = ref.fitness(D, m)
+        got = oracle.fitness(_ref_gpu_cfg(t.FLAG_REL_FITNESS | t.FLAG_FITNESS_PER_EXC), D, m)
+        assert abs(got - want) <= 2.4e-4 * abs(want), (got, want)
+        for wrong in (t.FLAG_REL_FITNESS, t.FLAG_FITNESS_PER_EXC):
+            bad = oracle.fitness(_ref_gpu_cfg(wrong), D, m)
+            assert abs(bad - want) > 0.5 * abs(want), (wrong, bad, want)
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (no reference tree)")
+def test_gpu_fitness_at_unit_measurement_is_compiled_fitness_over_n_freq(oracle):
+    """FDTD_GPU.cu:1004-1008, -sum (D - m)^2 / (n_exc n_freq), reduced to the
+    compiled fitness() of FDTD_common.cxx:53-61, -sum (m - D)^2 / (n_exc m^2),
+    at m = 1: the two differ by exactly the factor 1 / n_freq.  What stays
+    pinned by the source alone is that factor (FDTD_GPU.cu:1008 divides by
+    numberoffrequencies as well as numberofexcitationangles) and the absence
+    of m^2, which m = 1 cannot see."""
+    ref = Reference()
+    cfg = _ref_gpu_cfg(0)
+    for seed in (4, 5):
+        D, _ = _fitness_case(seed, 0.2, 1.8)
+        ones = np.ones_like(D)
+        want = ref.fitness(D, ones) / cfg.n_freq
+        got = oracle.fitness(cfg, D, ones)
+        assert abs(got - want) <= 2.4e-4 * abs(want), (got, want)
+
+
+def test_cpu_fitness_matches_compiled_fdtd_cpu_golden_value(oracle):
+    """The value the reference's whole FDTD_CPU() returned in a fresh process
+    (tests/golden/ref_fdtd_cpu_fitness.json): its far-field loop never runs
+    (FDTD_CPU.cxx:816), so it is the fitness of FDTD_CPU.cxx:841-845 at D = 0,
+    -sum (0 - m)^2 / (m^2 n_exc n_freq) accumulated in float, = -100.0038.
+    The oracle's FP64 form at D = 0 gives -100 exactly (4000 / 40), within
+    the float accumulation bound 4000 * ulp(100) / 2 = 0.0153 of it; dividing
+    by n_exc n_freq without m^2, or by m^2 n_exc alone, misses by orders of
+    magnitude.  The float accumulator restated on the same measurement gives
+    the golden value bit for bit."""
+    import json
+    with open(os.path.join(GOLD, "ref_fdtd_cpu_fitness.json")) as f:
+        gold = json.load(f)
+    cfg = ref_cpu_full()
+    m = t.measurement("cpu")[:cfg.farfield_len()]
+    D0 = np.zeros(cfg.farfield_len())
+    got = oracle.fitness(cfg, D0, m)
+    assert abs(got - (-100.0)) <= 1e-9
+    assert abs(got - gold["fitness"]) <= 4000 * 2.0 ** -17 / 2, (got, gold["fitness"])
+    for wrong in (cfg.flags & ~t.FLAG_REL_FITNESS, cfg.flags | t.FLAG_FITNESS_PER_EXC):
+        bad = oracle.fitness(cfg.copy(flags=wrong), D0, m)
+        assert abs(bad - gold["fitness"]) > 50, (wrong, bad)
+    # the reference's float accumulator: fit -= (double) pow(D - m, 2) / (pow(m, 2) * 4 * 10)
+    acc = np.float32(0.0)
+    for x in m.astype(np.float64):
+        acc = np.float32(np.float64(acc) - (0.0 - x) ** 2 / (x ** 2 * cfg.n_exc * cfg.n_freq))
+    assert float(acc) == gold["fitness"]
+
+
+def test_phase_fitness_term_special_cases(oracle):
+    """The phase term sketched at FDTD_GPU.cu:1009-1018: with D equal to the
+    target, each entry adds -dphi^2 / (10 pi^2 n_exc n_freq) with dphi = P -
+    P_target wrapped once into [-pi, pi] (dphi > pi: -2 pi; dphi < -pi: +2 pi).
+    Offsets 0.3, -2.0, 3 pi / 2 (-> -pi / 2) and -3 pi / 2 (-> pi / 2)."""
+    cfg = _ref_gpu_cfg(t.FLAG_PHASE_FITNESS)
+    L = cfg.farfield_len()
+    n = cfg.n_exc * cfg.n_freq
+    rng = np.random.default_rng(9)
+    D = rng.uniform(0.01, 0.05, L).astype(np.float32)
+    tp = rng.uniform(-1.0, 1.0, L).astype(np.float32)
+    for off, wrapped in ((0.3, 0.3), (-2.0, -2.0), (1.5 * PI, -0.5 * PI), (-1.5 * PI, 0.5 * PI)):
+        P = tp.astype(np.float64) + off
+        got = oracle.fitness(cfg, D, D, P=P, target_phase=tp)
+        want = -L * wrapped ** 2 / (10 * PI ** 2 * n)
+        assert abs(got - want) <= 1e-6 * abs(want), (off, got, want)
+    # without the flag the phase is ignored
+    assert oracle.fitness(_ref_gpu_cfg(0), D, D, P=tp.astype(np.float64) + 1.0, target_phase=tp) == 0.0
 
 
 FAT_FREQS = (1.5e9, 2.0e9, 2.5e9, 3.0e9, 3.5e9, 4.0e9)

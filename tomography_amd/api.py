@@ -58,11 +58,12 @@ class Stats(_c.Structure):
     """tomo_stats (include/tomo.h): one record per kernel (TOMO_KERNEL_*) and
     one for the whole update phase."""
 
-    _fields_ = [("kernel", KernelStats * len(KERNEL_NAMES)), ("update", KernelStats)]
+    _fields_ = [("kernel", KernelStats * len(KERNEL_NAMES)), ("update", KernelStats), ("requests", _c.c_int64)]
 
     def as_dict(self):
         d = {name: self.kernel[i].as_dict() for i, name in enumerate(KERNEL_NAMES)}
         d["update"] = self.update.as_dict()
+        d["requests"] = self.requests
         return d
 
 
@@ -156,17 +157,13 @@ class Context:
     """A device context: the owner of resident simulations and the request loop."""
 
     def __init__(self, device=0, hbm_budget=0):
-        import weakref
         h = _c.c_void_p()
         _check(lib().tomo_ctx_create(device, hbm_budget, _c.byref(h)))
         self.h = h
-        self._sims = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
-            for s in list(self._sims):  # simulations die with their context
-                s.close()
-            lib().tomo_ctx_destroy(self.h)
+            lib().tomo_ctx_destroy(self.h)  # frees the slots of live Sim handles too
             self.h = None
 
     def __del__(self):
@@ -303,12 +300,11 @@ class Sim:
         h = _c.c_void_p()
         _check(lib().tomo_sim_create(ctx.h, _c.byref(cfg), _p(p, _dp), p.size, exc, _c.byref(h)))
         self.h = h
-        ctx._sims.add(self)
 
     def close(self):
         if getattr(self, "h", None):
-            if self.ctx.h:  # a closed context already released the simulation's memory
-                lib().tomo_sim_destroy(self.h)
+            # safe in either order: after tomo_ctx_destroy the call frees the handle alone
+            lib().tomo_sim_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -59,6 +59,18 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_dropin_caller():
+    """tests/_bin/dropin_caller: a C++ caller of FDTD_GPU/FDTD_CPU shaped like the
+    reference's search driver, linked against libtomo.so (host code only)."""
+    out_dir = os.path.join(ROOT, "tests", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "dropin_caller")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", os.path.join(ROOT, "tests", "dropin_caller.cpp"), "-L" + HERE,
+          "-l:libtomo.so", "-Wl,-rpath,$ORIGIN/../../tomography_amd", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -66,6 +78,7 @@ def build_oracle():
 def main():
     build_oracle()
     print(build_library(verbose=True))
+    print(build_dropin_caller())
 
 
 if __name__ == "__main__":

@@ -42,6 +42,20 @@ double now_s() {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Device buffer freed when it leaves scope (so early error returns leak nothing).
+struct DevMem {
+    void* p = nullptr;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+    template <class X>
+    X* as() const { return (X*)p; }
+};
+
 struct Slot {
     SimSetup su;
     DevSim d{};
@@ -81,6 +95,14 @@ size_t slot_bytes(const SimSetup& s, int chunk) {
 }
 
 }  // namespace
+
+// A tomo_sim handle and its context share this record.  Destroying the context
+// first frees the simulation's slot and clears `ctx`, so a later call on the
+// handle sees a dead context instead of touching freed memory.
+struct SimLink {
+    tomo_ctx* ctx = nullptr;
+    Slot* s = nullptr;
+};
 
 struct tomo_request {
     uint64_t ticket = 0;
@@ -132,6 +154,8 @@ struct tomo_ctx {
     std::deque<uint64_t> queue;
     std::map<uint64_t, std::unique_ptr<tomo_request>> reqs;
     std::vector<Slot*> active;
+    // simulations created by tomo_sim_create (their slots are in neither `active` nor a request)
+    std::vector<std::shared_ptr<SimLink>> sims;
     // physical-mode targets, keyed by configuration bytes
     std::map<std::string, std::vector<float>> targets;
     // device memory of destroyed slots, kept for reuse by size
@@ -206,6 +230,8 @@ void release_cache(tomo_ctx* c) {
     c->cached_bytes = 0;
 }
 
+void slot_destroy(tomo_ctx* c, Slot* s);
+
 // Build the device-resident simulation for one excitation of one phantom.
 int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t np, int exc, Slot** out) {
     std::unique_ptr<Slot> s(new Slot());
@@ -229,6 +255,15 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     }
     s->bytes = bytes;
     c->used += bytes;
+    // every error return from here on hands the memory, the budget and the
+    // table index back through slot_destroy
+    struct Undo {
+        tomo_ctx* c;
+        std::unique_ptr<Slot>& s;
+        ~Undo() {
+            if (s) slot_destroy(c, s.release());
+        }
+    } undo{c, s};
     rc = ensure_table(c, 1);
     if (rc) return rc;
     if (c->free_idx.empty()) {
@@ -849,13 +884,14 @@ static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* para
         rc = get_target(c, cfg, tgt);
         if (rc) return rc;
     }
-    float *dD = nullptr, *dP = nullptr, *dT = nullptr;
-    double* dF = nullptr;
     const size_t nrec = n_vec * L;
-    HIPCHK(hipMalloc(&dD, nrec * 4 + 256));
-    HIPCHK(hipMalloc(&dP, nrec * 4 + 256));
-    HIPCHK(hipMalloc(&dT, 2 * L * 4 + 256));
-    HIPCHK(hipMalloc(&dF, n_vec * 8 + 256));
+    DevMem mD, mP, mT, mF;
+    HIPCHK(mD.alloc(nrec * 4 + 256));
+    HIPCHK(mP.alloc(nrec * 4 + 256));
+    HIPCHK(mT.alloc(2 * L * 4 + 256));
+    HIPCHK(mF.alloc(n_vec * 8 + 256));
+    float *dD = mD.as<float>(), *dP = mP.as<float>(), *dT = mT.as<float>();
+    double* dF = mF.as<double>();
     HIPCHK(hipMemcpyAsync(dT, tgt.data(), L * 4, hipMemcpyHostToDevice, c->stream));
     if (tph) HIPCHK(hipMemcpyAsync(dT + L, tph, L * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -920,10 +956,8 @@ static int eval_many(tomo_ctx* c, const tomo_sim_config& cfg, const double* para
             if (fit_out) fit_out[v] = hf[v];
         }
     }
-    (void)hipFree(dD);
-    (void)hipFree(dP);
-    (void)hipFree(dT);
-    (void)hipFree(dF);
+    // the copies above completed (or failed) before the buffers are released
+    (void)hipStreamSynchronize(c->stream);
     return rc;
 }
 
@@ -1078,9 +1112,17 @@ static int progress_once(tomo_ctx* c, int* n_active) {
 
 // ---------------------------------------------------------------- single simulation access
 struct tomo_sim {
-    tomo_ctx* ctx;
-    Slot* s;
+    std::shared_ptr<SimLink> link;
 };
+
+// Context and slot of a live handle, or an error when its context was destroyed.
+static int sim_live(tomo_sim* sim, tomo_ctx** c, Slot** s) {
+    if (!sim || !sim->link) return fail(TOMO_E_INVALID, "null sim");
+    if (!sim->link->ctx) return fail(TOMO_E_STATE, "the simulation's context was destroyed");
+    *c = sim->link->ctx;
+    *s = sim->link->s;
+    return TOMO_OK;
+}
 
 // ================================================================ C ABI
 extern "C" {
@@ -1128,11 +1170,21 @@ int tomo_ctx_destroy(tomo_ctx* c) {
     if (!c) return TOMO_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (Slot* s : c->active) slot_destroy(c, s);
+    // Every resident slot belongs to exactly one owner: a request (active or
+    // finished-but-unprojected) or a tomo_sim handle.  Free each once.
     c->active.clear();
-    for (auto& kv : c->reqs)
-        for (Slot* s : kv.second->slots)
-            if (std::find(c->active.begin(), c->active.end(), s) == c->active.end()) slot_destroy(c, s);
+    for (auto& kv : c->reqs) {
+        for (Slot* s : kv.second->slots) slot_destroy(c, s);
+        kv.second->slots.clear();
+    }
+    c->reqs.clear();
+    c->queue.clear();
+    for (auto& link : c->sims) {
+        slot_destroy(c, link->s);
+        link->s = nullptr;
+        link->ctx = nullptr;  // the handle now reports a destroyed context
+    }
+    c->sims.clear();
     if (c->d_sims) (void)hipFree(c->d_sims);
     if (c->h_items) (void)hipHostFree(c->h_items);
     if (c->d_items) (void)hipFree(c->d_items);
@@ -1303,6 +1355,7 @@ int tomo_get_stats(tomo_ctx* c, tomo_stats* out) {
     std::lock_guard<std::mutex> g(c->mu);
     int rc = drain_timings(c);
     *out = c->stats;
+    out->requests = (int64_t)c->reqs.size();
     return rc;
 }
 
@@ -1362,7 +1415,10 @@ int tomo_poll(tomo_ctx* c, uint64_t ticket, double* fitness_out, double* latency
     if (it == c->reqs.end()) return fail(TOMO_E_INVALID, "unknown ticket");
     tomo_request* r = it->second.get();
     if (r->state == 0 || r->state == 1) return TOMO_E_PENDING;
-    if (r->state == 4) return fail(TOMO_E_CANCELLED, "request cancelled");
+    if (r->state == 4) {  // reported once, then the record is gone like a finished one's
+        c->reqs.erase(it);
+        return fail(TOMO_E_CANCELLED, "request cancelled");
+    }
     if (fitness_out) *fitness_out = r->fitness;
     if (latency_s) *latency_s = r->t_done - r->t_submit;
     const int st = r->status;
@@ -1420,37 +1476,54 @@ int tomo_sim_create(tomo_ctx* c, const tomo_sim_config* cfg, const double* param
     Slot* s = nullptr;
     int rc = slot_create(c, *cfg, params, n, exc, &s);
     if (rc) return rc;
-    *out = new tomo_sim{c, s};
+    std::shared_ptr<SimLink> link = std::make_shared<SimLink>();
+    link->ctx = c;
+    link->s = s;
+    c->sims.push_back(link);
+    *out = new tomo_sim{link};
     return TOMO_OK;
 }
 
 int tomo_sim_destroy(tomo_sim* sim) {
     if (!sim) return TOMO_OK;
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    slot_destroy(sim->ctx, sim->s);
+    tomo_ctx* c = sim->link ? sim->link->ctx : nullptr;
+    if (c) {  // else the context was destroyed first and already freed the slot
+        std::lock_guard<std::mutex> g(c->mu);
+        slot_destroy(c, sim->link->s);
+        sim->link->s = nullptr;
+        sim->link->ctx = nullptr;
+        c->sims.erase(std::remove(c->sims.begin(), c->sims.end(), sim->link), c->sims.end());
+    }
     delete sim;
     return TOMO_OK;
 }
 
 int tomo_sim_step(tomo_sim* sim, int nsteps) {
-    if (!sim || nsteps < 0) return fail(TOMO_E_INVALID, "bad argument");
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    Slot* s = sim->s;
+    if (nsteps < 0) return fail(TOMO_E_INVALID, "bad argument");
+    tomo_ctx* c = nullptr;
+    Slot* s = nullptr;
+    int rc = sim_live(sim, &c, &s);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c->mu);
     s->target = std::min(s->su.cfg.steps, s->done + nsteps);
     std::vector<Slot*> one{s};
-    int rc = run_epochs(sim->ctx, one);
+    rc = run_epochs(c, one);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(sim->ctx->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return TOMO_OK;
 }
 
 int tomo_sim_fields(tomo_sim* sim, float* ez, float* hx, float* hy, float* jp) {
-    if (!sim) return fail(TOMO_E_INVALID, "null sim");
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    Slot* s = sim->s;
+    tomo_ctx* c = nullptr;
+    Slot* s = nullptr;
+    {
+        const int rc0 = sim_live(sim, &c, &s);
+        if (rc0) return rc0;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
     const DevSim& d = s->d;
     const int nx = d.nx, ny = d.ny, par = s->parity;
-    hipStream_t st = sim->ctx->stream;
+    hipStream_t st = c->stream;
     if (ez) HIPCHK(hipMemcpy2DAsync(ez, (nx + 1) * 4, d.ez[par], d.pitch * 4, (nx + 1) * 4, ny + 1, hipMemcpyDeviceToHost, st));
     if (hx) HIPCHK(hipMemcpy2DAsync(hx, nx * 4, d.hx[par], d.pitch * 4, nx * 4, ny, hipMemcpyDeviceToHost, st));
     if (hy) HIPCHK(hipMemcpy2DAsync(hy, nx * 4, d.hy[par], d.pitch * 4, nx * 4, ny, hipMemcpyDeviceToHost, st));
@@ -1469,9 +1542,13 @@ int tomo_sim_fields(tomo_sim* sim, float* ez, float* hx, float* hy, float* jp) {
 }
 
 int tomo_sim_materials(tomo_sim* sim, uint8_t* mat, float* coef, int* n_mat) {
-    if (!sim) return fail(TOMO_E_INVALID, "null sim");
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    Slot* s = sim->s;
+    tomo_ctx* c = nullptr;
+    Slot* s = nullptr;
+    {
+        const int rc0 = sim_live(sim, &c, &s);
+        if (rc0) return rc0;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
     const DevSim& d = s->d;
     if (mat)
         HIPCHK(hipMemcpy2D(mat, d.nx + 1, d.mat, d.pitch, d.nx + 1, d.ny + 1, hipMemcpyDeviceToHost));
@@ -1481,9 +1558,13 @@ int tomo_sim_materials(tomo_sim* sim, uint8_t* mat, float* coef, int* n_mat) {
 }
 
 int tomo_sim_dft(tomo_sim* sim, float* J, float* M) {
-    if (!sim) return fail(TOMO_E_INVALID, "null sim");
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    Slot* s = sim->s;
+    tomo_ctx* c = nullptr;
+    Slot* s = nullptr;
+    {
+        const int rc0 = sim_live(sim, &c, &s);
+        if (rc0) return rc0;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
     const DevSim& d = s->d;
     const size_t n = (size_t)d.nfreq * d.P;
     const bool f64 = (d.flags & TOMO_FLAG_DFT_FP64) != 0;
@@ -1505,15 +1586,20 @@ int tomo_sim_dft(tomo_sim* sim, float* J, float* M) {
 }
 
 int tomo_sim_farfield(tomo_sim* sim, float* D, float* P) {
-    if (!sim || !D) return fail(TOMO_E_INVALID, "null argument");
-    std::lock_guard<std::mutex> g(sim->ctx->mu);
-    Slot* s = sim->s;
+    if (!D) return fail(TOMO_E_INVALID, "null argument");
+    tomo_ctx* c = nullptr;
+    Slot* s = nullptr;
+    {
+        const int rc0 = sim_live(sim, &c, &s);
+        if (rc0) return rc0;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
     const size_t L = (size_t)s->su.cfg.n_freq * s->su.cfg.n_angles;
     float *dD = nullptr, *dP = nullptr;
     HIPCHK(hipMalloc(&dD, L * 4));
     HIPCHK(hipMalloc(&dP, L * 4));
     std::vector<Slot*> one{s};
-    int rc = project(sim->ctx, one, {0}, dD, dP);
+    int rc = project(c, one, {0}, dD, dP);
     if (!rc) {
         if (hipMemcpy(D, dD, L * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(TOMO_E_HIP, "copy");
         if (!rc && P && hipMemcpy(P, dP, L * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(TOMO_E_HIP, "copy");

@@ -168,8 +168,11 @@ extern "C" int tomo_search_run(tomo_ctx* ctx, const tomo_sim_config* cfg, const 
             it = out.erase(it);
         }
     }
-    // drain what is still resident (only after an error)
-    for (auto& kv : out) (void)tomo_cancel(ctx, kv.first);
+    // drain what is still resident (only after an error): cancel, then poll
+    // once so that the context drops the request records
+    for (auto& kv : out) {
+        if (tomo_cancel(ctx, kv.first) == TOMO_OK) (void)tomo_poll(ctx, kv.first, nullptr, nullptr);
+    }
     if (rc) return rc;
     if (gbest.empty()) return TOMO_E_NONFINITE;
     std::copy(gbest.begin(), gbest.end(), best_x);
