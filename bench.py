@@ -100,8 +100,26 @@ def traffic_for(workload, kernel, schedule):
     return rec.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
+def add_traffic(rl, traffic, tsteps):
+    """Counter-measured HBM bytes per launch beside the algorithmic figure:
+    the real HBM fraction, and the time-blocked minimum (a launch advances
+    tsteps steps, so the least it can move is the algorithmic bytes / tsteps,
+    the state read and written once)."""
+    if not rl:
+        return rl
+    rl["traffic"] = traffic
+    tmin = rl["alg_bytes_per_launch"] / tsteps
+    rl["blocked_min_bytes"] = round(tmin, 1)
+    if traffic:
+        avg_s = rl["avg_launch_us"] / 1e6
+        rl["traffic_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+        rl["traffic_over_blocked_min"] = round(traffic / tmin, 3)
+    return rl
+
+
 def kernel_ms(st, per=1.0):
-    return {n: round(st[n]["ms"] / per, 3) for n in st if n != "update" and st[n]["launches"]}
+    return {n: round(st[n]["ms"] / per, 3) for n in st
+            if isinstance(st[n], dict) and n != "update" and st[n]["launches"]}
 
 
 def apply_schedule(ctx, cfg, n_sims, sched, tune):
@@ -475,8 +493,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref", action="store_true", help="skip the reference-compatible (drop-in) leg")
-    ap.add_argument("--serve", type=int, default=0, help="requests of the continuous-batching study (per rate)")
-    ap.add_argument("--rate", type=str, default="2", help="Poisson arrival rates (requests/s), comma separated")
+    ap.add_argument("--serve", type=int, default=48, help="requests of the continuous-batching study (per rate; "
+                    "10000 in its definition, bounded by default; 0 = skip)")
+    ap.add_argument("--rate", type=str, default="3,6", help="Poisson arrival rates (requests/s), comma separated")
     ap.add_argument("--serve-plain-sample", type=int, default=8, help="plain-path requests timed on the serving mix")
     ap.add_argument("--search", type=int, default=0, help="evaluations of the asynchronous DE search driver")
     ap.add_argument("--schedule", type=str, default="",
@@ -486,7 +505,8 @@ def main():
     ap.add_argument("--plain-evals", type=int, default=2,
                     help="plain-path evals on the batched workload (for the batched/plain ratio; 0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
-    ap.add_argument("--uneven", type=int, default=0, help="requests of the uneven-batch study (512 in its definition)")
+    ap.add_argument("--uneven", type=int, default=128,
+                    help="requests of the uneven-batch study (512 in its definition, bounded by default; 0 = skip)")
     ap.add_argument("--uneven-plain-sample", type=int, default=2,
                     help="plain-path evals timed per grid-size bin of the uneven mix")
     ap.add_argument("--precision", type=int, default=0, help="results of the FP32-Kahan vs FP64 precision study")
@@ -515,7 +535,7 @@ def main():
            "ms_per_step_profiled_pass": round(dt_prof / args.steps * 1e3, 3)}
     rl = roofline_from(st)
     if rl:
-        rl["traffic"] = traffic_for("plain_cfg2", rl["kernel"], tuple(sched.values()))
+        add_traffic(rl, traffic_for("plain_cfg2", rl["kernel"], tuple(sched.values())), sched["tsteps"])
     out["roofline"] = rl
     out["kernel_ms_per_step"] = kernel_ms(st, args.steps)
     bsch = dict(BATCH_SCHEDULE)
@@ -526,7 +546,7 @@ def main():
         bcs = cell_steps(bcfg) * nvec
         brl = roofline_from(bst)
         if brl:
-            brl["traffic"] = traffic_for("batched_cfg3", brl["kernel"], tuple(bsched.values()))
+            add_traffic(brl, traffic_for("batched_cfg3", brl["kernel"], tuple(bsched.values())), bsched["tsteps"])
         out["batched"] = {"workload": "512^2, dx 0.5 mm, 4096 steps; 64 vectors x 16 antennas = 1024 simulations, "
                                       "16 freq x 360 angles (CFG3)",
                           "Mcell_steps_per_s": round(bcs / bdt / 1e6, 1), "evals_per_s": round(nvec / bdt, 3),

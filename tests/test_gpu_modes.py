@@ -86,6 +86,23 @@ def test_context_destroyed_with_resident_and_finished_requests():
         assert c2.stats()["requests"] == 0
 
 
+def test_request_cancelled_while_queued_leaves_the_queue():
+    """A request cancelled before admission, then collected by tomo_poll (which
+    erases its record), is gone from the admission queue too: the next epoch
+    admits and serves the other requests normally."""
+    cfg = phys_small(steps=96, n=96, n_exc=1)
+    with t.Context() as ctx:
+        want = ctx.eval(cfg, PHYS_PARAMS)
+        tk_cancel = ctx.submit(cfg, PHYS_PARAMS + [1, 0, 0, 0, 0, 0, 0, 0])
+        tk = ctx.submit(cfg, PHYS_PARAMS)
+        ctx.cancel(tk_cancel)
+        with pytest.raises(t.TomoError) as e:
+            ctx.poll(tk_cancel)
+        assert e.value.code == t.TOMO_E_CANCELLED
+        assert ctx.wait(tk)[0] == want
+        assert ctx.stats()["requests"] == 0
+
+
 def test_fp64_dft_mode_vs_fp64_oracle(ctx, oracle):
     """TOMO_FLAG_DFT_FP64 (FP64 phases and accumulators) in both physics
     modes: J, M and D against the FP64 oracle within 1e-4, and within 1e-6 of

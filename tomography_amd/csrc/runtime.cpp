@@ -104,6 +104,31 @@ struct SimLink {
     Slot* s = nullptr;
 };
 
+namespace {
+// Launch staging of the epoch runner: launch items and the tile list (host
+// copies and device copies) and the DevSim table the kernels index.  The
+// context's own staging grows on demand (pinned host memory, device memory);
+// a fixed staging lives in caller-provided memory and never grows.
+struct Staging {
+    LaunchItem* h_items = nullptr;
+    LaunchItem* d_items = nullptr;
+    int items_cap = 0;
+    TileItem* h_tiles = nullptr;
+    TileItem* d_tiles = nullptr;
+    int tiles_cap = 0;
+    DevSim* d_sims = nullptr;
+    // host staging may be rewritten once this completes (growable staging);
+    // a fixed staging waits for its stream instead
+    hipEvent_t staged = nullptr;
+    bool fixed = false;
+    std::vector<uint64_t> tile_key;  // serials of the simulations the uploaded tile list was built for
+    int key_T = 0, key_S = 0, key_W = 0, key_Wi = 0, key_ER = 0, key_Si = 0;
+    int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
+    int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
+    int n_corner = 0;
+};
+}  // namespace
+
 struct tomo_request {
     uint64_t ticket = 0;
     tomo_sim_config cfg{};
@@ -133,20 +158,9 @@ struct tomo_ctx {
     DevSim* d_sims = nullptr;
     int cap = 0;
     std::vector<int> free_idx;
-    // epoch staging
-    LaunchItem* h_items = nullptr;
-    LaunchItem* d_items = nullptr;
-    int items_cap = 0;
-    TileItem* h_tiles = nullptr;
-    TileItem* d_tiles = nullptr;
-    int tiles_cap = 0;
-    hipEvent_t staged = nullptr;
-    std::vector<uint64_t> tile_key;  // serials of the simulations the uploaded tile list was built for
+    // epoch staging (its d_sims is the table above)
+    Staging stg;
     uint64_t next_serial = 1;
-    int tile_key_T = 0, tile_key_S = 0, tile_key_W = 0, tile_key_Wi = 0, tile_key_ER = 0, tile_key_Si = 0;
-    int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
-    int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
-    int n_corner = 0;
     int Wi = 4;                  // waves per workgroup of the interior kernel
     // serving
     std::mutex mu;
@@ -199,25 +213,36 @@ int ensure_table(tomo_ctx* c, int need) {
     return TOMO_OK;
 }
 
-int ensure_staging(tomo_ctx* c, int nitems, int ntiles) {
-    if (nitems > c->items_cap) {
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if (c->h_items) HIPCHK(hipHostFree(c->h_items));
-        if (c->d_items) HIPCHK(hipFree(c->d_items));
-        int cap = std::max(nitems, 2 * c->items_cap);
-        HIPCHK(hipHostMalloc(&c->h_items, sizeof(LaunchItem) * cap, hipHostMallocDefault));
-        HIPCHK(hipMalloc(&c->d_items, sizeof(LaunchItem) * cap));
-        c->items_cap = cap;
+int ensure_staging(tomo_ctx* c, Staging& st, int nitems, int ntiles) {
+    if (st.fixed) {
+        if (nitems > st.items_cap || ntiles > st.tiles_cap)
+            return fail(TOMO_E_NOMEM, "launch items or tiles exceed the fixed staging");
+        return TOMO_OK;
     }
-    if (ntiles > c->tiles_cap) {
+    if (nitems > st.items_cap) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (c->h_tiles) HIPCHK(hipHostFree(c->h_tiles));
-        if (c->d_tiles) HIPCHK(hipFree(c->d_tiles));
-        int cap = std::max(ntiles, 2 * c->tiles_cap);
-        HIPCHK(hipHostMalloc(&c->h_tiles, sizeof(TileItem) * cap, hipHostMallocDefault));
-        HIPCHK(hipMalloc(&c->d_tiles, sizeof(TileItem) * cap));
-        c->tiles_cap = cap;
-        c->tile_key.clear();
+        if (st.h_items) HIPCHK(hipHostFree(st.h_items));
+        if (st.d_items) HIPCHK(hipFree(st.d_items));
+        const int cap = std::max(nitems, 2 * st.items_cap);
+        st.h_items = nullptr;
+        st.d_items = nullptr;
+        st.items_cap = 0;
+        HIPCHK(hipHostMalloc(&st.h_items, sizeof(LaunchItem) * cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&st.d_items, sizeof(LaunchItem) * cap));
+        st.items_cap = cap;
+    }
+    if (ntiles > st.tiles_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (st.h_tiles) HIPCHK(hipHostFree(st.h_tiles));
+        if (st.d_tiles) HIPCHK(hipFree(st.d_tiles));
+        const int cap = std::max(ntiles, 2 * st.tiles_cap);
+        st.h_tiles = nullptr;
+        st.d_tiles = nullptr;
+        st.tiles_cap = 0;
+        HIPCHK(hipHostMalloc(&st.h_tiles, sizeof(TileItem) * cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&st.d_tiles, sizeof(TileItem) * cap));
+        st.tiles_cap = cap;
+        st.tile_key.clear();
     }
     return TOMO_OK;
 }
@@ -231,6 +256,7 @@ void release_cache(tomo_ctx* c) {
 }
 
 void slot_destroy(tomo_ctx* c, Slot* s);
+int slot_fill(tomo_ctx* c, Slot* s, DevSim* d_entry);
 
 // Build the device-resident simulation for one excitation of one phantom.
 int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t np, int exc, Slot** out) {
@@ -273,7 +299,21 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
     s->idx = c->free_idx.back();
     c->free_idx.pop_back();
     s->serial = c->next_serial++;
+    rc = slot_fill(c, s.get(), c->d_sims + s->idx);
+    if (rc) return rc;
+    s->exc = exc;
+    s->target = su.cfg.steps;
+    *out = s.release();
+    return TOMO_OK;
+}
 
+// Lay out a slot in s->mem (s->bytes), zero its state and accumulators, upload
+// its tables and rasterise its material map, and write its DevSim to d_entry
+// (all on the context's stream; the uploads read host memory owned by the
+// slot, which must outlive them).
+int slot_fill(tomo_ctx* c, Slot* s, DevSim* d_entry) {
+    const SimSetup& su = s->su;
+    const size_t bytes = s->bytes;
     const tomo_sim_config& cf = su.cfg;
     const size_t rows = (size_t)cf.ny + 1, pitch = su.pitch, nc = cf.ncpml, P = su.P;
     char* m = s->mem;
@@ -380,12 +420,9 @@ int slot_create(tomo_ctx* c, const tomo_sim_config& cfg, const double* p, size_t
         HIPCHK(hipMemcpyAsync(yc, su.ref.yc.data(), rows * 4, hipMemcpyHostToDevice, st));
         HIPCHK(launch_raster_ref(mat, su.pitch, cf.nx, cf.ny, lo, hi, xc, yc, cf.numpatches, st));
     }
-    HIPCHK(hipMemcpyAsync(c->d_sims + s->idx, &s->d, sizeof(DevSim), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_entry, &s->d, sizeof(DevSim), hipMemcpyHostToDevice, st));
     // the uploads above read host memory owned by the slot, which outlives them
     // (slot_destroy waits for the stream before releasing it)
-    s->exc = exc;
-    s->target = cf.steps;
-    *out = s.release();
     return TOMO_OK;
 }
 
@@ -545,8 +582,15 @@ struct LaunchTimer {
 // Epochs whose launch items one upload carries, and the item count it may hold.
 constexpr int kMaxStagedEpochs = 256, kMaxStagedItems = 65536;
 
-// Run epochs until every slot in `sl` reaches its target step.
-int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
+// The context's own staging, over its current DevSim table.
+Staging& stage(tomo_ctx* c) {
+    c->stg.d_sims = c->d_sims;
+    return c->stg;
+}
+
+// Run epochs until every slot in `sl` reaches its target step, staging the
+// launches in `st` (whose d_sims holds the slots' DevSim entries).
+int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
     const int T = c->T, S = c->S, W = c->W, K = c->K;
     if (!tblock_supported(T) || K % T || K > 64 || W < 1 || W > 4 || S < 1)
         return fail(TOMO_E_INVALID, "unsupported schedule");
@@ -563,29 +607,30 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             std::vector<uint64_t> key;
             for (Slot* s : ms) key.push_back(s->serial);
             std::vector<TileItem> te, tr, tc, tn;
-            const bool rebuild = key != c->tile_key || c->tile_key_T != T || c->tile_key_S != S ||
-                                 c->tile_key_W != W || c->tile_key_Wi != c->Wi || c->tile_key_ER != c->ER ||
-                                 c->tile_key_Si != c->Si;
+            const bool rebuild = key != st.tile_key || st.key_T != T || st.key_S != S || st.key_W != W ||
+                                 st.key_Wi != c->Wi || st.key_ER != c->ER || st.key_Si != c->Si;
             if (rebuild) {
                 for (size_t i = 0; i < ms.size(); ++i)
                     build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
             }
-            const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : c->ntiles;
+            const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : st.ntiles;
             // Items of E consecutive epochs in one upload: every simulation of
             // the set is active in each of them (only the last may be partial),
             // so the set, and with it the tile list, stays fixed.  A captured
             // graph reads one epoch's items at a fixed address, so graphs stage one.
             const int nsim = (int)ms.size();
-            int E = c->use_graph ? 1 : std::max(1, std::min(kMaxStagedEpochs, kMaxStagedItems / nsim));
+            int E = (c->use_graph && !st.fixed) ? 1 : std::max(1, std::min(kMaxStagedEpochs, kMaxStagedItems / nsim));
             for (Slot* s : ms) E = std::min(E, (s->target - s->done + K - 1) / K);
-            int rc = ensure_staging(c, E * nsim, ntiles);
+            int rc = ensure_staging(c, st, E * nsim, ntiles);
             if (rc) return rc;
-            HIPCHK(hipEventSynchronize(c->staged));
+            // the host staging is rewritten below: wait for its last upload
+            if (st.staged) HIPCHK(hipEventSynchronize(st.staged));
+            else HIPCHK(hipStreamSynchronize(c->stream));
             int maxP = 0, maxF = 0;
             for (int i = 0; i < nsim; ++i) {
                 Slot* s = ms[i];
                 for (int e = 0; e < E; ++e) {
-                    LaunchItem& it = c->h_items[e * nsim + i];
+                    LaunchItem& it = st.h_items[e * nsim + i];
                     it.sim = s->idx;
                     it.step0 = s->done + e * K;
                     it.parity0 = s->parity ^ ((e * (K / T)) & 1);  // epochs before the last are full
@@ -594,34 +639,34 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 maxP = std::max(maxP, s->d.P);
                 maxF = std::max(maxF, s->d.nfreq);
             }
-            HIPCHK(hipMemcpyAsync(c->d_items, c->h_items, sizeof(LaunchItem) * E * nsim, hipMemcpyHostToDevice,
+            HIPCHK(hipMemcpyAsync(st.d_items, st.h_items, sizeof(LaunchItem) * E * nsim, hipMemcpyHostToDevice,
                                   c->stream));
             if (rebuild) {
-                TileItem* h = c->h_tiles;
+                TileItem* h = st.h_tiles;
                 h = std::copy(te.begin(), te.end(), h);
                 h = std::copy(tr.begin(), tr.end(), h);
                 h = std::copy(tc.begin(), tc.end(), h);
                 std::copy(tn.begin(), tn.end(), h);
-                HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, sizeof(TileItem) * ntiles, hipMemcpyHostToDevice,
+                HIPCHK(hipMemcpyAsync(st.d_tiles, st.h_tiles, sizeof(TileItem) * ntiles, hipMemcpyHostToDevice,
                                       c->stream));
-                c->tile_key = key;
-                c->tile_key_T = T; c->tile_key_S = S; c->tile_key_W = W; c->tile_key_Wi = c->Wi;
-                c->tile_key_ER = c->ER;
-                c->tile_key_Si = c->Si;
-                c->ntiles = ntiles;
-                c->n_edge = (int)(te.size() + tr.size() + tc.size());
-                c->n_corner = (int)te.size();
-                c->n_rowedge = (int)tr.size();
+                st.tile_key = key;
+                st.key_T = T; st.key_S = S; st.key_W = W; st.key_Wi = c->Wi;
+                st.key_ER = c->ER;
+                st.key_Si = c->Si;
+                st.ntiles = ntiles;
+                st.n_edge = (int)(te.size() + tr.size() + tc.size());
+                st.n_corner = (int)te.size();
+                st.n_rowedge = (int)tr.size();
             }
-            HIPCHK(hipEventRecord(c->staged, c->stream));
-            const int n_edge = c->n_edge, n_inner = c->ntiles - c->n_edge;
+            if (st.staged) HIPCHK(hipEventRecord(st.staged, c->stream));
+            const int n_edge = st.n_edge, n_inner = st.ntiles - st.n_edge;
             const int cap = c->cus * c->grid_mult;
             // the staged epoch being launched: its items on the host and the device
-            const LaunchItem* h_it = c->h_items;
-            LaunchItem* d_it = c->d_items;
+            const LaunchItem* h_it = st.h_items;
+            LaunchItem* d_it = st.d_items;
             int nl = 0;  // time blocks of that epoch
             // tile classes: corners (general kernel), row-edge, column-edge, interior
-            const int n0 = c->n_corner, nr = c->n_rowedge, ncl = n_edge - n0 - nr;
+            const int n0 = st.n_corner, nr = st.n_rowedge, ncl = n_edge - n0 - nr;
             const int cnt[3] = {n0, nr, ncl};
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
@@ -652,16 +697,16 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     }
                     if (mode != TOMO_MODE_PHYSICAL) {
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tblock(T, mode, c->d_tiles, n_edge, d_it, c->d_sims, o, W,
+                        HIPCHK(launch_tblock(T, mode, st.d_tiles, n_edge, d_it, st.d_sims, o, W,
                                              std::min(n_edge, cap), c->stream));
                         lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
                         if (!first) first = lt.a;
                         continue;
                     }
                     if (fused) {
-                        const int nt = c->ntiles;
+                        const int nt = st.ntiles;
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tfused(T, c->d_tiles, nt, d_it, c->d_sims, o, std::max(W, c->Wi),
+                        HIPCHK(launch_tfused(T, st.d_tiles, nt, d_it, st.d_sims, o, std::max(W, c->Wi),
                                              std::min(nt, 2 * cap), c->stream));
                         lt.done(TOMO_KERNEL_FUSED, cells[0] + cells[1] + cells[2] + cells[3],
                                 bytes[0] + bytes[1] + bytes[2] + bytes[3]);
@@ -672,22 +717,22 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                     int off = 0;
                     for (int k = 0; k < 3; ++k) {
                         const int n = cnt[k];
-                        const TileItem* tl = c->d_tiles + off;
+                        const TileItem* tl = st.d_tiles + off;
                         off += n;
                         if (n == 0) continue;
                         hipStream_t es = multi ? c->side[k] : c->stream;
                         if (multi) HIPCHK(hipStreamWaitEvent(es, c->ev_bar, 0));
                         LaunchTimer lt(c);
-                        if (k == 0) HIPCHK(launch_tedge(T, tl, n, d_it, c->d_sims, o, W, std::min(n, 2 * cap), es));
-                        if (k == 1) HIPCHK(launch_trow(T, tl, n, d_it, c->d_sims, o, c->Wi, std::min(n, 2 * cap), es));
-                        if (k == 2) HIPCHK(launch_tcol(T, tl, n, d_it, c->d_sims, o, W, std::min(n, 2 * cap), es));
+                        if (k == 0) HIPCHK(launch_tedge(T, tl, n, d_it, st.d_sims, o, W, std::min(n, 2 * cap), es));
+                        if (k == 1) HIPCHK(launch_trow(T, tl, n, d_it, st.d_sims, o, c->Wi, std::min(n, 2 * cap), es));
+                        if (k == 2) HIPCHK(launch_tcol(T, tl, n, d_it, st.d_sims, o, W, std::min(n, 2 * cap), es));
                         lt.done(TOMO_KERNEL_CORNER + k, cells[k], bytes[k]);
                         if (!first) first = lt.a;
                         if (multi) HIPCHK(hipEventRecord(c->ev_side[k], es));
                     }
                     if (n_inner > 0) {
                         LaunchTimer lt(c);
-                        HIPCHK(launch_tinner(T, c->d_tiles + n_edge, n_inner, d_it, c->d_sims, o, c->Wi,
+                        HIPCHK(launch_tinner(T, st.d_tiles + n_edge, n_inner, d_it, st.d_sims, o, c->Wi,
                                              std::min(n_inner, 2 * cap), c->stream));
                         lt.done(TOMO_KERNEL_INNER, cells[3], bytes[3]);
                         if (!first) first = lt.a;
@@ -703,7 +748,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
                 for (int prec = 0; prec < 2; ++prec) {
                     if (!(prec ? fp64_any : fp32_any)) continue;
                     LaunchTimer lt(c);
-                    HIPCHK(launch_fold(prec == 1, d_it, (int)ms.size(), c->d_sims, maxP, maxF, c->stream));
+                    HIPCHK(launch_fold(prec == 1, d_it, (int)ms.size(), st.d_sims, maxP, maxF, c->stream));
                     lt.done(TOMO_KERNEL_FOLD);
                     if (!fold_start) fold_start = lt.a;
                 }
@@ -714,18 +759,18 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl) {
             // hipGraph and replayed (timed runs launch directly: their per-launch
             // events cannot live inside a graph).
             for (int e = 0; e < E; ++e) {
-                h_it = c->h_items + e * nsim;
-                d_it = c->d_items + e * nsim;
+                h_it = st.h_items + e * nsim;
+                d_it = st.d_items + e * nsim;
                 int maxn = 0, minn = K;
                 for (int i = 0; i < nsim; ++i) {
                     maxn = std::max(maxn, h_it[i].nsteps);
                     minn = std::min(minn, h_it[i].nsteps);
                 }
                 nl = (maxn + T - 1) / T;
-                if (c->use_graph && !c->prof && minn == K) {
+                if (c->use_graph && !st.fixed && !c->prof && minn == K) {
                     const std::vector<int64_t> gk = {mode, T, S, c->Si, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
                                                      fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)d_it,
-                                                     (int64_t)(intptr_t)c->d_tiles, (int64_t)(intptr_t)c->d_sims};
+                                                     (int64_t)(intptr_t)st.d_tiles, (int64_t)(intptr_t)st.d_sims};
                     if (!c->gexec || gk != c->gkey || key != c->gkey_serials) {
                         if (c->gexec) {
                             HIPCHK(hipGraphExecDestroy(c->gexec));
@@ -791,9 +836,12 @@ static int check_uniform_precision(const std::vector<Slot*>& sl) {
 // offset out_off[i]).  Consecutive simulations that share grid, contour,
 // frequencies and angles and whose records are contiguous form one group: one
 // matrix-core launch per group.
+// The kernels index `d_sims` (the context's table by default); `staging`, when
+// given, holds the launch's small tables instead of a device allocation.
 static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<int>& out_off, float* dD,
-                   float* dP) {
+                   float* dP, const DevSim* d_sims = nullptr, char* staging = nullptr, size_t staging_cap = 0) {
     if (sl.empty()) return TOMO_OK;
+    if (!d_sims) d_sims = c->d_sims;
     auto same = [](const tomo_sim_config& x, const tomo_sim_config& y) {
         return x.mode == y.mode && x.nx == y.nx && x.ny == y.ny && x.ntff_offset == y.ntff_offset &&
                x.dx == y.dx && x.dy == y.dy && x.n_freq == y.n_freq && x.n_angles == y.n_angles &&
@@ -828,8 +876,9 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
             kinv.push_back(make_float2((float)(2 * kPI * su.freq[f] / c0()), (float)su.inv_norm[f]));
     }
     const size_t bytes = align256(idx.size() * sizeof(int)) + kinv.size() * sizeof(float2);
-    char* d = nullptr;
-    HIPCHK(hipMalloc(&d, bytes));
+    char* d = staging;
+    if (d && bytes > staging_cap) return fail(TOMO_E_NOMEM, "projection tables exceed the fixed staging");
+    if (!d) HIPCHK(hipMalloc(&d, bytes));
     int* d_idx = (int*)d;
     float2* d_k = (float2*)(d + align256(idx.size() * sizeof(int)));
     hipError_t e = hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
@@ -839,13 +888,13 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
         const Group& g = groups[gi];
         const tomo_sim_config& cf = sl[g.i0]->su.cfg;
         LaunchTimer lt(c);
-        e = launch_project_mfma(d_idx + g.i0, (int)g.n, cf.n_freq, c->d_sims, d_k + kofs[gi], cf.n_angles,
+        e = launch_project_mfma(d_idx + g.i0, (int)g.n, cf.n_freq, d_sims, d_k + kofs[gi], cf.n_angles,
                                 (float)eta0(), dD + out_off[g.i0], dP ? dP + out_off[g.i0] : nullptr, c->stream);
         if (e == hipSuccess) lt.done(TOMO_KERNEL_PROJECT);
     }
     // the uploads read host vectors of this frame
     const hipError_t es = hipStreamSynchronize(c->stream);
-    (void)hipFree(d);
+    if (!staging) (void)hipFree(d);
     HIPCHK(e);
     HIPCHK(es);
     return TOMO_OK;
@@ -856,7 +905,7 @@ static int project(tomo_ctx* c, const std::vector<Slot*>& sl, const std::vector<
 static int run_and_project(tomo_ctx* c, const std::vector<Slot*>& sl, float* dD, float* dP) {
     int rc = check_uniform_precision(sl);
     if (rc) return rc;
-    rc = run_epochs(c, sl);
+    rc = run_epochs(c, sl, stage(c));
     if (rc) return rc;
     std::vector<int> off;
     for (size_t i = 0; i < sl.size(); ++i) off.push_back((int)(i * sl[i]->su.cfg.n_freq * sl[i]->su.cfg.n_angles));
@@ -1049,8 +1098,9 @@ static int finish_request(tomo_ctx* c, tomo_request* r) {
 static int progress_once(tomo_ctx* c, int* n_active) {
     // admission by HBM footprint, FIFO
     while (!c->queue.empty()) {
-        tomo_request* r = c->reqs[c->queue.front()].get();
-        if (r->state == 4) { c->queue.pop_front(); continue; }
+        auto qi = c->reqs.find(c->queue.front());
+        tomo_request* r = qi == c->reqs.end() ? nullptr : qi->second.get();
+        if (!r || r->state == 4) { c->queue.pop_front(); continue; }
         if (c->used + (uint64_t)r->bytes > c->budget) {
             if (c->active.empty()) {  // cannot ever fit
                 r->state = 3;
@@ -1086,22 +1136,29 @@ static int progress_once(tomo_ctx* c, int* n_active) {
             s->target = std::min(s->su.cfg.steps, s->done + c->K);
             run.push_back(s);
         }
-        int rc = run_epochs(c, run);
+        int rc = run_epochs(c, run, stage(c));
         if (rc) return rc;
         // retire
         std::vector<Slot*> still;
         std::vector<tomo_request*> fin;
+        // every active slot belongs to a running request, whose record tomo_poll
+        // keeps (it reports running requests as pending)
+        auto owner = [&](const Slot* s) -> tomo_request* {
+            auto it = c->reqs.find((uint64_t)s->req);
+            return it == c->reqs.end() ? nullptr : it->second.get();
+        };
         for (Slot* s : c->active) {
             if (s->done >= s->su.cfg.steps) {
-                tomo_request* r = c->reqs[(uint64_t)s->req].get();
+                tomo_request* r = owner(s);
+                if (!r) continue;
                 bool all = true;
                 for (Slot* q : r->slots) all = all && q->done >= q->su.cfg.steps;
                 if (all && std::find(fin.begin(), fin.end(), r) == fin.end()) fin.push_back(r);
             }
         }
         for (Slot* s : c->active) {
-            tomo_request* r = c->reqs[(uint64_t)s->req].get();
-            if (std::find(fin.begin(), fin.end(), r) == fin.end()) still.push_back(s);
+            tomo_request* r = owner(s);
+            if (r && std::find(fin.begin(), fin.end(), r) == fin.end()) still.push_back(s);
         }
         c->active.swap(still);
         for (tomo_request* r : fin) finish_request(c, r);
@@ -1158,8 +1215,8 @@ int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx** out) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     c->budget = hbm_budget ? hbm_budget : (uint64_t)(0.9 * (double)fr);
-    HIPCHK(hipEventCreateWithFlags(&c->staged, hipEventDisableTiming));
-    HIPCHK(hipEventRecord(c->staged, c->stream));
+    HIPCHK(hipEventCreateWithFlags(&c->stg.staged, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->stg.staged, c->stream));
     int rc = ensure_table(c.get(), 256);
     if (rc) return rc;
     *out = c.release();
@@ -1186,11 +1243,11 @@ int tomo_ctx_destroy(tomo_ctx* c) {
     }
     c->sims.clear();
     if (c->d_sims) (void)hipFree(c->d_sims);
-    if (c->h_items) (void)hipHostFree(c->h_items);
-    if (c->d_items) (void)hipFree(c->d_items);
-    if (c->h_tiles) (void)hipHostFree(c->h_tiles);
-    if (c->d_tiles) (void)hipFree(c->d_tiles);
-    if (c->staged) (void)hipEventDestroy(c->staged);
+    if (c->stg.h_items) (void)hipHostFree(c->stg.h_items);
+    if (c->stg.d_items) (void)hipFree(c->stg.d_items);
+    if (c->stg.h_tiles) (void)hipHostFree(c->stg.h_tiles);
+    if (c->stg.d_tiles) (void)hipFree(c->stg.d_tiles);
+    if (c->stg.staged) (void)hipEventDestroy(c->stg.staged);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     (void)drain_timings(c);
     release_cache(c);
@@ -1243,12 +1300,12 @@ static int time_epochs(tomo_ctx* c, std::vector<Slot*>& sl, int epochs, double* 
     hipEvent_t a = pool_event(c), b = pool_event(c);
     if (!a || !b) return fail(TOMO_E_HIP, "event");
     for (Slot* s : sl) s->target = std::min(s->su.cfg.steps, s->done + c->K);
-    int rc = run_epochs(c, sl);  // untimed: rebuilds and uploads the tile list
+    int rc = run_epochs(c, sl, stage(c));  // untimed: rebuilds and uploads the tile list
     if (rc) return rc;
     HIPCHK(hipEventRecord(a, c->stream));
     for (int e = 0; e < epochs; ++e) {
         for (Slot* s : sl) s->target = std::min(s->su.cfg.steps, s->done + c->K);
-        rc = run_epochs(c, sl);
+        rc = run_epochs(c, sl, stage(c));
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(b, c->stream));
@@ -1327,7 +1384,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     if (!rc) rc = set_schedule_locked(c, cur);
     c->prof = prof;
     for (Slot* s : sl) slot_destroy(c, s);
-    c->tile_key.clear();
+    c->stg.tile_key.clear();
     if (rc) return rc;
     if (best) *best = cur;
     if (rate) *rate = cells / best_t;
@@ -1451,6 +1508,8 @@ int tomo_cancel(tomo_ctx* c, uint64_t ticket) {
     if (it == c->reqs.end()) return fail(TOMO_E_INVALID, "unknown ticket");
     tomo_request* r = it->second.get();
     if (r->state == 0) {
+        // leaves the admission queue now: tomo_poll may erase the record before the next epoch
+        c->queue.erase(std::remove(c->queue.begin(), c->queue.end(), ticket), c->queue.end());
         r->state = 4;
         r->t_done = now_s();
         return TOMO_OK;
@@ -1507,7 +1566,7 @@ int tomo_sim_step(tomo_sim* sim, int nsteps) {
     std::lock_guard<std::mutex> g(c->mu);
     s->target = std::min(s->su.cfg.steps, s->done + nsteps);
     std::vector<Slot*> one{s};
-    rc = run_epochs(c, one);
+    rc = run_epochs(c, one, stage(c));
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return TOMO_OK;
