@@ -217,6 +217,26 @@ int tomo_eval(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
               double *fitness_out, float *D_out, float *P_out);
 
 /*
+ * No counterpart in the reference.  A variant of tomo_eval for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments and the same
+ * results as tomo_eval, bit for bit, with two limits: `target` is required
+ * (physical mode: fetch it once beforehand with tomo_target_farfield), and one
+ * excitation's simulation plus its tables must fit TOMO_NOALLOC_SCRATCH_BYTES
+ * (grids up to 1024^2 do; larger ones return TOMO_E_NOMEM).  The call makes no
+ * device or pinned-host allocation: every device buffer of the evaluation
+ * lives in one array compiled into the library, one copy per device, held
+ * until the process ends.  Calls on one device share that array (which keeps
+ * the last call's contents; what must start at zero is cleared each call), so
+ * overlapping calls would race: the library serialises them per device within
+ * a process.  Per-launch timing (tomo_profile) is not recorded inside it.
+ */
+#define TOMO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tomo_eval_noalloc(tomo_ctx *ctx, const tomo_sim_config *cfg, const double *params,
+                      size_t n_params, const float *target, size_t n_target,
+                      double *fitness_out, float *D_out, float *P_out);
+
+/*
  * Batched evaluation of n_vec parameter vectors (row-major, `dim` each) with
  * a shared configuration: all n_vec * n_exc simulations (as many as the HBM
  * budget admits at once) advance together in the same time-blocked launches.

@@ -34,6 +34,7 @@ PRESET_CFG4 = 5
 PRESET_CFG5 = 6
 
 MAX_MATERIALS = 128
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 FORM_STREAMS = 1
 FORM_FUSED = 2

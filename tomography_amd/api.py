@@ -25,7 +25,7 @@ EXPORTS = [
     "tomo_target_farfield", "tomo_submit", "tomo_poll", "tomo_wait", "tomo_cancel", "tomo_progress",
     "tomo_request_bytes", "tomo_sim_create", "tomo_sim_destroy", "tomo_sim_step", "tomo_sim_fields",
     "tomo_sim_materials", "tomo_sim_dft", "tomo_sim_farfield", "tomo_profile", "tomo_use_graphs", "tomo_get_stats",
-    "tomo_search_run", "tomo_fitness",
+    "tomo_search_run", "tomo_fitness", "tomo_eval_noalloc",
 ]
 
 
@@ -114,6 +114,7 @@ def lib():
     L.tomo_autotune.argtypes = [vp, vp, _c.c_int, vp, _dp]
     L.tomo_eval.argtypes = [vp, vp, _dp, _c.c_size_t, _fp, _c.c_size_t, _dp, _fp, _fp]
     L.tomo_eval_batch.argtypes = [vp, vp, _dp, _c.c_size_t, _c.c_size_t, _fp, _c.c_size_t, _dp, _fp]
+    L.tomo_eval_noalloc.argtypes = [vp, vp, _dp, _c.c_size_t, _fp, _c.c_size_t, _dp, _fp, _fp]
     L.tomo_target_farfield.argtypes = [vp, vp, _fp]
     L.tomo_submit.argtypes = [vp, vp, _dp, _c.c_size_t, _c.POINTER(_c.c_uint64)]
     L.tomo_poll.argtypes = [vp, _c.c_uint64, _dp, _dp]
@@ -235,6 +236,18 @@ class Context:
         fit = _c.c_double()
         _check(lib().tomo_eval(self.h, _c.byref(cfg), _p(p, _dp), p.size, _p(t, _fp), 0 if t is None else t.size,
                                _c.byref(fit), _p(D, _fp), _p(P, _fp)))
+        return (fit.value, D, P) if want_farfield else fit.value
+
+    def eval_noalloc(self, cfg, params, target, want_farfield=False):
+        """tomo_eval without device allocation (the target is required)."""
+        p = np.ascontiguousarray(params, np.float64)
+        t = np.ascontiguousarray(target, np.float32)
+        L = cfg.farfield_len()
+        D = np.zeros(L, np.float32) if want_farfield else None
+        P = np.zeros(L, np.float32) if want_farfield else None
+        fit = _c.c_double()
+        _check(lib().tomo_eval_noalloc(self.h, _c.byref(cfg), _p(p, _dp), p.size, _p(t, _fp), t.size, _c.byref(fit),
+                                       _p(D, _fp), _p(P, _fp)))
         return (fit.value, D, P) if want_farfield else fit.value
 
     def eval_batch(self, cfg, params, target=None, want_farfield=False):

@@ -143,6 +143,7 @@ struct tomo_request {
 struct tomo_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    char* scratch = nullptr;  // device address of the allocation-free entry's array on this device
     int cus = 256;
     uint64_t budget = 0, used = 0;
     int T = 4, S = 32, W = 4, K = 32;  // schedule (W: waves per edge workgroup)
@@ -1167,6 +1168,126 @@ static int progress_once(tomo_ctx* c, int* n_active) {
     return TOMO_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free evaluation
+// Device memory of tomo_eval_noalloc: every buffer of its evaluation lives
+// here, so that the entry allocates nothing.  Each device the library runs on
+// holds its own copy of the array until the process ends.  Host code never
+// uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) char scratch[TOMO_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                // guards g_scratch_ptr
+char* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device
+std::mutex g_noalloc_mu[kMaxDevices];   // one allocation-free evaluation at a time per device
+
+// Device address of `scratch` on `device`, which must be current.
+int scratch_address(int device, char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(TOMO_E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(TOMO_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (char*)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return TOMO_OK;
+}
+
+// Small tables of the projection launch: simulation indices and per
+// frequency {k, normaliser} (one simulation at a time here).
+constexpr size_t kNoallocProjBytes = 256 + TOMO_MAX_FREQ * sizeof(float2);
+
+// tomo_eval's plain path with every device buffer carved from `base`
+// (TOMO_NOALLOC_SCRATCH_BYTES): fixed tables first, the simulation slot at the
+// end, the tile list in between.  The caller holds the context's and the
+// device's locks.
+int eval_noalloc(tomo_ctx* c, char* base, const tomo_sim_config& cfg, const double* params, size_t n_params,
+                 const float* target, size_t n_target, double* fit_out, float* D_out, float* P_out) {
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!params) return fail(TOMO_E_INVALID, "null parameters");
+    const int F = cfg.n_freq, A = cfg.n_angles, E = cfg.n_exc;
+    const size_t L = (size_t)E * F * A;
+    if (!target || n_target < L)
+        return fail(TOMO_E_INVALID, "tomo_eval_noalloc needs the target far field (n_exc*n_freq*n_angles values)");
+    const float* tph = ((cfg.flags & TOMO_FLAG_PHASE_FITNESS) && n_target >= 2 * L) ? target + L : nullptr;
+    if (c->K % c->T) return fail(TOMO_E_INVALID, "chunk must be a multiple of the fused step count");
+    // host setup of every excitation first: it sizes the slot
+    std::vector<std::unique_ptr<Slot>> slots;
+    size_t sbytes = 0;
+    for (int e = 0; e < E; ++e) {
+        std::unique_ptr<Slot> s(new Slot());
+        rc = build_setup(cfg, params, n_params, e, s->su);
+        if (rc) return rc;
+        sbytes = std::max(sbytes, slot_bytes(s->su, c->K));
+        slots.push_back(std::move(s));
+    }
+    size_t off = 0;
+    auto carve = [&](size_t b) { char* r = base + off; off += align256(b); return r; };
+    DevSim* tbl = (DevSim*)carve(sizeof(DevSim));
+    float* dD = (float*)carve(L * 4);
+    float* dP = (float*)carve(L * 4);
+    float* dT = (float*)carve(2 * L * 4);
+    double* dF = (double*)carve(sizeof(double));
+    char* pstage = carve(kNoallocProjBytes);
+    LaunchItem* d_items = (LaunchItem*)carve(sizeof(LaunchItem) * kMaxStagedEpochs);
+    const size_t total = TOMO_NOALLOC_SCRATCH_BYTES;
+    const size_t slot_off = (total - sbytes) & ~(size_t)255;
+    if (sbytes > total || slot_off < off + align256(64 * sizeof(TileItem)))
+        return fail(TOMO_E_NOMEM, "configuration needs " + std::to_string(off + sbytes) +
+                                      " bytes of scratch; tomo_eval_noalloc holds " + std::to_string(total));
+    TileItem* d_tiles = (TileItem*)(base + off);
+    const int tiles_cap = (int)std::min<size_t>((slot_off - off) / sizeof(TileItem), 1u << 24);
+    char* slot_mem = base + slot_off;
+    // pageable host staging: a fixed staging waits for the stream before rewriting it
+    std::vector<LaunchItem> h_items(kMaxStagedEpochs);
+    std::vector<TileItem> h_tiles((size_t)tiles_cap);
+    Staging st;
+    st.fixed = true;
+    st.h_items = h_items.data();
+    st.d_items = d_items;
+    st.items_cap = kMaxStagedEpochs;
+    st.h_tiles = h_tiles.data();
+    st.d_tiles = d_tiles;
+    st.tiles_cap = tiles_cap;
+    st.d_sims = tbl;
+    hipStream_t sm = c->stream;
+    HIPCHK(hipMemcpyAsync(dT, target, L * 4, hipMemcpyHostToDevice, sm));
+    if (tph) HIPCHK(hipMemcpyAsync(dT + L, tph, L * 4, hipMemcpyHostToDevice, sm));
+    for (int e = 0; e < E; ++e) {
+        Slot* s = slots[e].get();
+        s->mem = slot_mem;  // each excitation reuses the slot memory after the previous one's work (stream order)
+        s->bytes = sbytes;
+        s->idx = 0;
+        s->serial = c->next_serial++;
+        rc = slot_fill(c, s, tbl);
+        if (rc) return rc;
+        s->exc = e;
+        s->target = cfg.steps;
+        const std::vector<Slot*> one{s};
+        rc = run_epochs(c, one, st);
+        if (rc) return rc;
+        rc = project(c, one, {0}, dD + (size_t)e * F * A, dP + (size_t)e * F * A, tbl, pstage, kNoallocProjBytes);
+        if (rc) return rc;
+    }
+    const int terms = (tph ? 1 : 0) | ((cfg.flags & TOMO_FLAG_REL_FITNESS) ? 2 : 0);
+    const double norm = (cfg.flags & TOMO_FLAG_FITNESS_PER_EXC) ? (double)E : (double)E * F;
+    HIPCHK(launch_fitness(dD, dP, dT, tph ? dT + L : nullptr, (int)L, 1, norm, terms, dF, sm));
+    double fit = 0;
+    HIPCHK(hipMemcpyAsync(&fit, dF, sizeof(double), hipMemcpyDeviceToHost, sm));
+    if (D_out) HIPCHK(hipMemcpyAsync(D_out, dD, L * 4, hipMemcpyDeviceToHost, sm));
+    if (P_out) HIPCHK(hipMemcpyAsync(P_out, dP, L * 4, hipMemcpyDeviceToHost, sm));
+    HIPCHK(hipStreamSynchronize(sm));
+    if (cfg.mode == TOMO_MODE_PHYSICAL && !std::isfinite(fit)) return fail(TOMO_E_NONFINITE, "non-finite fitness");
+    if (fit_out) *fit_out = fit;
+    return TOMO_OK;
+}
+}  // namespace
+
 // ---------------------------------------------------------------- single simulation access
 struct tomo_sim {
     std::shared_ptr<SimLink> link;
@@ -1202,6 +1323,9 @@ int tomo_ctx_create(int device, uint64_t hbm_budget, tomo_ctx** out) {
     HIPCHK(hipSetDevice(device));
     std::unique_ptr<tomo_ctx> c(new tomo_ctx());
     c->device = device;
+    // set-up of the allocation-free entry: this device's scratch address
+    int rs = scratch_address(device, &c->scratch);
+    if (rs) return rs;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int k = 0; k < 3; ++k) {
         HIPCHK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
@@ -1422,6 +1546,21 @@ int tomo_eval(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, siz
     std::lock_guard<std::mutex> g(c->mu);
     (void)hipSetDevice(c->device);
     return eval_many(c, *cfg, params, 1, n_params, target, n_target, fitness_out, D_out, P_out, true);
+}
+
+int tomo_eval_noalloc(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n_params,
+                      const float* target, size_t n_target, double* fitness_out, float* D_out, float* P_out) {
+    if (!c || !cfg) return fail(TOMO_E_INVALID, "null argument");
+    if (!c->scratch) return fail(TOMO_E_STATE, "no scratch address for this device");
+    std::lock_guard<std::mutex> g(c->mu);
+    std::lock_guard<std::mutex> gd(g_noalloc_mu[c->device]);
+    (void)hipSetDevice(c->device);
+    // per-launch timing would create events: it stays off inside this call
+    const bool prof = c->prof;
+    c->prof = false;
+    const int rc = eval_noalloc(c, c->scratch, *cfg, params, n_params, target, n_target, fitness_out, D_out, P_out);
+    c->prof = prof;
+    return rc;
 }
 
 int tomo_eval_batch(tomo_ctx* c, const tomo_sim_config* cfg, const double* params, size_t n_vec, size_t dim,
