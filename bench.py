@@ -49,7 +49,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")
+UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")
 
 
 def _kernel_roofline(name, k):
@@ -459,6 +459,47 @@ def run_ref_compat(ctx, sched, n_eval=2, ref_steps=250):
     return out
 
 
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "fdtd_cpu_main")
+
+
+def start_reference_full_run():
+    """The reference as shipped: the whole FDTD_CPU() call (FDTD_CPU.cxx:669-892,
+    compiled unmodified into oracle/_ref/fdtd_cpu_main; 401^2, 4 excitations x
+    2000 steps) once on one host core, in its own process, started before the
+    GPU legs and timed beside them."""
+    if not os.path.exists(REF_MAIN):
+        return None
+    rng = np.random.default_rng(20261208)
+    v = np.concatenate([rng.uniform(1, 7, 81), rng.uniform(0, 0.3, 81)])
+    p = subprocess.Popen([REF_MAIN], stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                         text=True)
+    t0 = time.perf_counter()
+    p.stdin.write(" ".join(repr(float(x)) for x in v))
+    p.stdin.close()
+    p.stdin = None  # communicate() later only reads
+    return p, t0
+
+
+def finish_reference_full_run(run, limit_s=420):
+    if run is None:
+        return None
+    p, t0 = run
+    try:
+        out, _ = p.communicate(timeout=max(1.0, limit_s - (time.perf_counter() - t0)))
+    except subprocess.TimeoutExpired:
+        p.kill()
+        p.communicate()
+        return {"status": f"stopped after {limit_s} s"}
+    dt = time.perf_counter() - t0
+    lines = out.strip().splitlines()
+    cfg = t.preset(t.PRESET_REF_CPU)
+    return {"kind": "reference", "cores": 1, "s_per_eval": round(dt, 2), "evals_per_s": round(1 / dt, 5),
+            "Mcell_steps_per_s": round(cell_steps(cfg) / dt / 1e6, 2),
+            "returned": lines[-1].split()[-1] if lines else None,
+            "sample": "the whole FDTD_CPU() call as shipped (oracle/_ref/fdtd_cpu_main: 401^2, 4 excitations x "
+                      "2000 steps), one process on one host core, run while the GPU legs ran"}
+
+
 def platform_host():
     model = ""
     try:
@@ -510,12 +551,23 @@ def main():
     ap.add_argument("--uneven-plain-sample", type=int, default=2,
                     help="plain-path evals timed per grid-size bin of the uneven mix")
     ap.add_argument("--precision", type=int, default=0, help="results of the FP32-Kahan vs FP64 precision study")
+    ap.add_argument("--ref-schedule", type=str, default="", help="drop-in leg schedule (same fields as --schedule)")
+    ap.add_argument("--only-ref", action="store_true",
+                    help="run only the reference-compatible (drop-in) leg and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(simulations are independent: more GPUs would each run their own share).", file=sys.stderr)
         return 2
+    ref_full = None if (args.no_ref or args.only_ref) else start_reference_full_run()
     ctx = t.Context()
+    rsched = dict(REF_SCHEDULE)
+    if args.ref_schedule:
+        rsched = dict(zip(SCHED_KEYS, [int(x) for x in args.ref_schedule.split(",")]))
+    if args.only_ref:
+        print(json.dumps({"ref_compat": run_ref_compat(ctx, rsched)}))
+        ctx.close()
+        return 0
     tune = args.autotune
     psched = dict(PLAIN_SCHEDULE)
     if args.schedule:
@@ -568,7 +620,13 @@ def main():
     if args.calib_copy_mb:
         hip_copy_calibration(args.calib_copy_mb)
     if not args.no_ref:
-        out["ref_compat"] = run_ref_compat(ctx, REF_SCHEDULE)
+        out["ref_compat"] = run_ref_compat(ctx, rsched)
+        full = finish_reference_full_run(ref_full)
+        if full:
+            out["ref_compat"]["reference_FDTD_CPU_full"] = full
+            if "s_per_eval" in full:
+                out["ref_compat"]["FDTD_CPU_gpu_vs_reference_cpu_full"] = round(
+                    full["s_per_eval"] / out["ref_compat"]["FDTD_CPU"]["s_per_eval"], 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["ratio_gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

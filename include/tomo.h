@@ -144,6 +144,13 @@ const char *tomo_last_error(void);
  * boundary tiles; at most seg_rows). */
 #define TOMO_FORM_STREAMS 1
 #define TOMO_FORM_FUSED 2
+/* TOMO_FORM_RESIDENT: one cooperative launch per DFT chunk in which each
+ * workgroup keeps one tile of a simulation (a 64-lane band, seg_rows rows per
+ * wave, 8 waves) in registers through every time block, exchanging only a
+ * T-deep frame with its neighbours; seg_rows 8 or 12, tsteps 2, 4 or 8.  Runs
+ * that hold more tiles than the device can keep co-resident use the fused
+ * form instead. */
+#define TOMO_FORM_RESIDENT 3
 typedef struct tomo_schedule {
     int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows;
     /* Output rows of an interior tile of a physical-mode simulation; 0 = seg_rows
@@ -177,7 +184,8 @@ enum {
     TOMO_KERNEL_PROJECT = 6, /* k_project_mfma: far-field projection on the matrix cores */
     TOMO_KERNEL_FITNESS = 7, /* k_fitness: FP64 fitness reduction */
     TOMO_KERNEL_FUSED = 8,   /* k_tfused: every tile class in one launch (TOMO_FORM_FUSED) */
-    TOMO_KERNEL_COUNT = 9
+    TOMO_KERNEL_RESIDENT = 9, /* k_tres: a whole epoch in one cooperative launch (TOMO_FORM_RESIDENT) */
+    TOMO_KERNEL_COUNT = 10
 };
 typedef struct tomo_kernel_stats {
     int64_t launches;

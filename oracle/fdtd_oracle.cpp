@@ -52,10 +52,13 @@
 //   Phase output P of N2FPostProcess: the same disc's complex far field
 //       sqrt(D) e^{jP} matches its series far field (with the reference's
 //       phase centre, half a cell off the disc centre for odd nx) within 1.5%.
-//   Parity unpinned (FDTD_GPU.cu cannot be compiled here and its known-answer
-//   candidate, the commented vector of FDTD_GPU.cu:659-692 against
-//   measurement[] of FDTD_GPU.cu:996, does not reproduce that array): the GPU
-//   fitness normalisation is restated from the source only.
+//   Fitness: fitness() of FDTD_common.cxx:53-61 against the compiled function
+//       on random arrays; FDTD_CPU.cxx:841-845 against the value the whole
+//       compiled FDTD_CPU() returns (D = 0, its far-field loop never runs);
+//       FDTD_GPU.cu:1004-1008 reduced to the compiled fitness() at m = 1,
+//       which leaves its 1/n_freq factor pinned by the source alone
+//       (FDTD_GPU.cu cannot be compiled here); the phase term of
+//       FDTD_GPU.cu:1009-1018 by its special cases.
 
 #include <cmath>
 #include <complex>

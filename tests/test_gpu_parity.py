@@ -90,11 +90,14 @@ def test_phys_fields_vs_oracle(ctx, oracle):
                                    (2, 8, 2, 32, 4, 2, t.FORM_FUSED), (8, 5, 4, 64, 1, 1, t.FORM_FUSED),
                                    (4, 16, 1, 32, 3, 8, t.FORM_FUSED), (8, 8, 4, 64, 4, 16, t.FORM_FUSED, 2),
                                    (4, 32, 1, 64, 4, 16, t.FORM_STREAMS, 3),
-                                   (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 8), (2, 5, 2, 32, 4, 4, t.FORM_STREAMS, 2, 12)])
+                                   (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 8), (2, 5, 2, 32, 4, 4, t.FORM_STREAMS, 2, 12),
+                                   (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 14),
+                                   (4, 8, 1, 32, 1, 16, t.FORM_RESIDENT), (8, 12, 1, 64, 1, 16, t.FORM_RESIDENT),
+                                   (2, 12, 1, 32, 1, 16, t.FORM_RESIDENT), (4, 12, 1, 64, 1, 16, t.FORM_RESIDENT)])
 def test_schedule_does_not_change_results(oracle, sched):
     """Fused step count, tile heights (boundary and interior), waves per
-    workgroup, DFT chunk and the launch form only reorder work: every schedule
-    gives the same fields and accumulators."""
+    workgroup, DFT chunk and the launch form (streams, fused, resident) only
+    reorder work: every schedule gives the same fields and accumulators."""
     cfg = phys_small(steps=200)
     base = t.Context()
     base.set_schedule(4, 32, 4, 32, form=t.FORM_STREAMS)
@@ -263,7 +266,7 @@ def test_autotune_installs_a_schedule_that_keeps_results(oracle):
     c.close()
 
 
-@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("form", [1, 2, 3])
 def test_kernel_stats_account_for_every_cell_step(form):
     """Per-kernel timing: the update kernels' cell-steps add up to the
     simulations' cell-steps, every tile class of a split grid runs (in its
@@ -271,19 +274,19 @@ def test_kernel_stats_account_for_every_cell_step(form):
     cell-step stay within the 25..49 bytes the formula allows."""
     cfg = phys_small(steps=96, n=256, n_exc=2)
     c = t.Context()
-    c.set_schedule(4, 16, 2, 32, 2, 2, form)
+    c.set_schedule(4, 16 if form != 3 else 8, 2, 32, 2, 2, form)
     c.target_farfield(cfg)  # the target's own simulations stay out of the counters
     c.profile(True)
     c.eval(cfg, PHYS_PARAMS)
     st = c.stats()
-    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")]
+    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")]
     want = float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
     assert sum(k["cell_steps"] for k in upd) == want
     assert st["update"]["cell_steps"] == want
-    ran = ("k_tedge", "k_trow", "k_tcol", "k_tinner") if form == 1 else ("k_tfused",)
+    ran = {1: ("k_tedge", "k_trow", "k_tcol", "k_tinner"), 2: ("k_tfused",), 3: ("k_tres",)}[form]
     for k in ran + ("k_fold", "k_project_mfma", "k_fitness"):
         assert st[k]["launches"] > 0 and st[k]["ms"] > 0, k
-    for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused")) - set(ran):
+    for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")) - set(ran):
         assert st[k]["launches"] == 0, k
     b = sum(k["alg_bytes"] for k in upd) / want
     assert 25.0 < b < 49.0
@@ -499,3 +502,31 @@ def test_ref_gpu_far_field_phase_of_debye_disc_matches_series(ctx):
     D, P = g.farfield()
     g.close()
     check_disc_far_field(D, P, cfg, mat)
+
+
+def test_resident_form_batches_requests_and_fallback_equal_other_forms():
+    """The resident form with several simulations in one cooperative launch
+    (a batch of vectors x antennas, and a request-loop mix of grid sizes)
+    gives exactly the fitness and far fields of the streams form; a grid whose
+    tiles cannot all be co-resident (2049 x 2049 nodes) falls back to the
+    fused launches and keeps the same fields."""
+    cfg = phys_small(steps=300, n=256, n_exc=2)
+    vecs = np.array([PHYS_PARAMS, PHYS_PARAMS + [1, 1, 0.5, 1, 2, 1, 0.1, 0], PHYS_PARAMS - [2, 0, 1, 2, 3, 2, 0.1, 0]])
+    mix = [(phys_small(steps=200, n=96, n_exc=1).copy(breast_radius=0.012, skin_thickness=0.001, src_radius=0.015),
+            PHYS_PARAMS), (phys_small(steps=260, n=160, n_exc=3), PHYS_PARAMS + [-2, 2, 1, 2, 2, 1, 0.1, 0])]
+    out = []
+    for form, rows in ((t.FORM_STREAMS, 16), (t.FORM_RESIDENT, 8)):
+        with t.Context() as c:
+            c.set_schedule(tsteps=4, seg_rows=rows, edge_waves=1, chunk=64, inner_waves=4, grid_mult=16, form=form)
+            fb, Db = c.eval_batch(cfg, vecs, want_farfield=True)
+            tks = [c.submit(cf, v) for cf, v in mix]
+            served = [c.wait(tk)[0] for tk in tks]
+            big = t.preset(t.PRESET_CFG4, 2048).copy(steps=64)
+            g = c.sim(big, PHYS_PARAMS, exc=0)
+            g.step(64)
+            ez = g.fields()[0]
+            g.close()
+            out.append((fb, Db, served, ez))
+    (fa, Da, sa, ea), (fr, Dr, sr, er) = out
+    assert np.array_equal(fa, fr) and np.array_equal(Da, Dr) and sa == sr
+    assert np.array_equal(ea, er) and np.abs(er).max() > 0

@@ -32,6 +32,18 @@ hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchI
 hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
                          int ordinal, int waves, int grid, hipStream_t st);
 
+// Resident update form: one cooperative launch advances every simulation of
+// `items` by its epoch (all time blocks), one workgroup per ResTile, all
+// co-resident.  `done` (ntiles counters) must start at zero; `err` is set to 1
+// if a workgroup gave up waiting (after spin_limit polls) and stays set.
+constexpr int kResidentWaves = 8;  // waves per workgroup of the resident kernel
+bool tres_supported(int T, int rows_per_wave);
+// Tiles one cooperative launch can hold on `cus` compute units (0: none).
+int tres_max_tiles(int T, int rows_per_wave, int mode, int cus);
+hipError_t launch_tres(int T, int rows_per_wave, int mode, const ResTile* tiles, int ntiles,
+                       const LaunchItem* items, const DevSim* sims, int* done, int* err, int spin_limit,
+                       hipStream_t st);
+
 // Fold one chunk of surface samples into the running DFT (Kahan FP32 or FP64).
 hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const DevSim* sims, int max_p, int max_f,
                        hipStream_t st);

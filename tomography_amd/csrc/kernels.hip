@@ -156,6 +156,21 @@ __device__ __forceinline__ bool row_ystrip_h(const SimView& S, int r) {
     return (r >= 0 && r < S.nc) || (r >= S.ny - S.nc && r < S.ny);
 }
 
+// Element access of load_cell / store_cell.  COH: relaxed atomics at agent
+// scope (coherent across the XCDs' L2 caches without a cache flush), for data
+// that another workgroup of the same launch wrote or will read.
+template <bool COH>
+__device__ __forceinline__ float ld_(gfloat* p) {
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void st_(gfloat* p, float v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+template <bool COH = false>
 __device__ __forceinline__ void load_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x, bool xin,
                                           const WaveFlags& wf, Cell& c) {
     c = zero_cell();
@@ -168,61 +183,62 @@ __device__ __forceinline__ void load_cell(const SimView& S, const float4* __rest
     if (!xin) return;
     const int base = r * S.pitch + x;
     const int in_f = S.par * S.ps;
-    c.ez = S.f[in_f + base];
+    c.ez = ld_<COH>(S.f + in_f + base);
     if (x < S.nx && r < S.ny) {
-        c.hx = S.f[in_f + 2 * S.ps + base];
-        c.hy = S.f[in_f + 4 * S.ps + base];
+        c.hx = ld_<COH>(S.f + in_f + 2 * S.ps + base);
+        c.hy = ld_<COH>(S.f + in_f + 4 * S.ps + base);
     }
     const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
     if (hi > lo && x >= lo && x < hi)
-        c.jp = S.jp[S.par * S.js + S.span[3 * r + 2] + x - lo];
+        c.jp = ld_<COH>(S.jp + S.par * S.js + S.span[3 * r + 2] + x - lo);
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
         gfloat* pe = S.xq + S.par * S.xs;
         gfloat* ph = S.xq + (2 + S.par) * S.xs;
-        if (x >= 1 && x <= nc) c.pex = pe[r * w + x - 1];
-        else if (x >= nx - nc && x <= nx - 1) c.pex = pe[r * w + nc + x - (nx - nc)];
-        if (x < nc) c.phx = ph[r * w + x];
-        else if (x >= nx - nc && x < nx) c.phx = ph[r * w + nc + x - (nx - nc)];
+        if (x >= 1 && x <= nc) c.pex = ld_<COH>(pe + r * w + x - 1);
+        else if (x >= nx - nc && x <= nx - 1) c.pex = ld_<COH>(pe + r * w + nc + x - (nx - nc));
+        if (x < nc) c.phx = ld_<COH>(ph + r * w + x);
+        else if (x >= nx - nc && x < nx) c.phx = ld_<COH>(ph + r * w + nc + x - (nx - nc));
     }
     if (row_ystrip_e(S, r)) {
         const int j = r <= nc ? r - 1 : nc + r - (ny - nc);
-        c.pey = S.yq[S.par * S.ys + j * S.pitch + x];
+        c.pey = ld_<COH>(S.yq + S.par * S.ys + j * S.pitch + x);
     }
     if (row_ystrip_h(S, r)) {
         const int j = r < nc ? r : nc + r - (ny - nc);
-        c.phy = S.yq[(2 + S.par) * S.ys + j * S.pitch + x];
+        c.phy = ld_<COH>(S.yq + (2 + S.par) * S.ys + j * S.pitch + x);
     }
 }
 
+template <bool COH = false>
 __device__ __forceinline__ void store_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x,
                                            const WaveFlags& wf, const Cell& c) {
     const int base = r * S.pitch + x;
     const int out_f = (S.par ^ 1) * S.ps;
-    S.f[out_f + base] = c.ez;
+    st_<COH>(S.f + out_f + base, c.ez);
     if (x < S.nx && r < S.ny) {
-        S.f[out_f + 2 * S.ps + base] = c.hx;
-        S.f[out_f + 4 * S.ps + base] = c.hy;
+        st_<COH>(S.f + out_f + 2 * S.ps + base, c.hx);
+        st_<COH>(S.f + out_f + 4 * S.ps + base, c.hy);
     }
     const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
     if (hi > lo && x >= lo && x < hi)
-        S.jp[(S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo] = c.jp;
+        st_<COH>(S.jp + (S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo, c.jp);
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
         gfloat* pe = S.xq + (S.par ^ 1) * S.xs;
         gfloat* ph = S.xq + (2 + (S.par ^ 1)) * S.xs;
-        if (x >= 1 && x <= nc) pe[r * w + x - 1] = c.pex;
-        else if (x >= nx - nc && x <= nx - 1) pe[r * w + nc + x - (nx - nc)] = c.pex;
-        if (x < nc) ph[r * w + x] = c.phx;
-        else if (x >= nx - nc && x < nx) ph[r * w + nc + x - (nx - nc)] = c.phx;
+        if (x >= 1 && x <= nc) st_<COH>(pe + r * w + x - 1, c.pex);
+        else if (x >= nx - nc && x <= nx - 1) st_<COH>(pe + r * w + nc + x - (nx - nc), c.pex);
+        if (x < nc) st_<COH>(ph + r * w + x, c.phx);
+        else if (x >= nx - nc && x < nx) st_<COH>(ph + r * w + nc + x - (nx - nc), c.phx);
     }
     if (row_ystrip_e(S, r)) {
         const int j = r <= nc ? r - 1 : nc + r - (ny - nc);
-        S.yq[(S.par ^ 1) * S.ys + j * S.pitch + x] = c.pey;
+        st_<COH>(S.yq + (S.par ^ 1) * S.ys + j * S.pitch + x, c.pey);
     }
     if (row_ystrip_h(S, r)) {
         const int j = r < nc ? r : nc + r - (ny - nc);
-        S.yq[(2 + (S.par ^ 1)) * S.ys + j * S.pitch + x] = c.phy;
+        st_<COH>(S.yq + (2 + (S.par ^ 1)) * S.ys + j * S.pitch + x, c.phy);
     }
 }
 
@@ -278,13 +294,10 @@ __device__ __forceinline__ int ytab_idx(int d, int nc) {
 __device__ __forceinline__ int ytab_e(int y, int ny, int nc) { return ytab_idx(min(y - 1, ny - 1 - y), nc); }
 __device__ __forceinline__ int ytab_h(int y, int ny, int nc) { return ytab_idx(min(y, ny - 1 - y), nc); }
 
-// One leapfrog step of one row: c holds row y at level n (fields Ez^n,
-// H^{n-1/2}); ez_up = Ez^n(x, y+1); hx_prev = Hx^{n+1/2}(x, y-1) on entry and
-// Hx^{n+1/2}(x, y) on exit.  On exit c holds row y at level n+1.
+// H half-step of stage_update: returns Hx, Hy at n + 1/2 (c's H psi advanced).
 template <int MODE>
-__device__ __forceinline__ void stage_update(const SimView& S, const float4* __restrict__ lc4,
-                                             const float* __restrict__ lcp, const WaveFlags& wf, Cell& c,
-                                             float ez_up, float& hx_prev, int x, int y, int n) {
+__device__ __forceinline__ void stage_h(const SimView& S, const float* __restrict__ lcp, const WaveFlags& wf,
+                                        Cell& c, float ez_up, int x, int y, float& hx_out, float& hy_out) {
     const int nx = S.nx, ny = S.ny, nc = S.nc;
     // ---------------- H half-step (H_field_update, FDTD_GPU.cu:298-355)
     const float ez_r = lane_next(c.ez);
@@ -347,6 +360,17 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float4* __r
         hx = 0.f;
         hy = 0.f;
     }
+    hx_out = hx;
+    hy_out = hy;
+}
+
+// E half-step of stage_update given the row's new H; hx_prev holds Hx of the
+// row below on entry and this row's Hx on exit.
+template <int MODE>
+__device__ __forceinline__ void stage_e(const SimView& S, const float4* __restrict__ lc4,
+                                        const float* __restrict__ lcp, const WaveFlags& wf, Cell& c, float hx,
+                                        float hy, float& hx_prev, int x, int y, int n) {
+    const int nx = S.nx, ny = S.ny, nc = S.nc;
     // ---------------- E half-step (E_field_update, FDTD_GPU.cu:358-434)
     const float hy_l = lane_prev(hy);
     const float dHy = hy - hy_l, dHx = hx - hx_prev;
@@ -422,6 +446,18 @@ __device__ __forceinline__ void stage_update(const SimView& S, const float4* __r
         c.jp = qb.y * c.jp + qb.z * (en - ez);
         c.ez = en;
     }
+}
+
+// One leapfrog step of one row: c holds row y at level n (fields Ez^n,
+// H^{n-1/2}); ez_up = Ez^n(x, y+1); hx_prev = Hx^{n+1/2}(x, y-1) on entry and
+// Hx^{n+1/2}(x, y) on exit.  On exit c holds row y at level n+1.
+template <int MODE>
+__device__ __forceinline__ void stage_update(const SimView& S, const float4* __restrict__ lc4,
+                                             const float* __restrict__ lcp, const WaveFlags& wf, Cell& c,
+                                             float ez_up, float& hx_prev, int x, int y, int n) {
+    float hx, hy;
+    stage_h<MODE>(S, lcp, wf, c, ez_up, x, y, hx, hy);
+    stage_e<MODE>(S, lc4, lcp, wf, c, hx, hy, hx_prev, x, y, n);
 }
 
 // Surface samples of the NTFF contour (contour map of FDTD_common.cxx:4-46):
@@ -742,6 +778,323 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
 }
 
 
+#undef CP
+
+// ---------------------------------------------------------------- resident update
+// One workgroup of kResWaves waves per tile (ResTile), all tiles of the launch
+// co-resident (cooperative launch).  Wave w keeps region rows
+// [y0 - T + w RW, y0 - T + (w + 1) RW) of a 64-lane band in registers for
+// the whole epoch, and every step advances them bottom to top with the same
+// stage arithmetic as the general edge kernel (select form), so every cell
+// gets bit-identical values in every launch form.  Per step the waves of a
+// workgroup couple through LDS: each publishes its bottom row's Ez^n, then
+// the H half-step of its top row (which needs the next wave's Ez^n), then all
+// rows run; two barriers per step.  Per time block of T steps, ghost lanes
+// and rows (stale after T steps, like a band's ghost zone) are reloaded from
+// the global planes, where the neighbouring tiles stored their T-deep frame
+// at the end of the previous block and published it through their
+// completed-block counter.  Frame stores, ghost loads and the counters are
+// relaxed atomics at agent scope, coherent across the XCDs' L2 caches, and
+// every frame store has completed (vmcnt 0, then a barrier) before the
+// counter is written, so no cache writeback or invalidation is needed.  Waits are bounded: past `spin_limit` polls, or once
+// any workgroup has given up, a workgroup raises the error word and returns,
+// so every wave exits.
+constexpr int kResWaves = 8;
+
+// Lane constants of a band: x-strip CPML coefficients (identity off the strips)
+// and the node masks of the column.
+struct ResLane {
+    float kxe, axe, bxe, kxh, axh, bxh;
+    bool hx_ok, ex_ok;
+};
+
+#define CP(k, i) lcp[(k) * TOMO_MAX_NC + (i)]
+// H half-step of one row (the select-form stage of body_edge): returns Hx, Hy
+// at n + 1/2 and advances the row's H psi.
+__device__ __forceinline__ void res_h(const SimView& S, const float* __restrict__ lcp, const ResLane& L, Cell& c,
+                                      float ez_up, int y, float& hx, float& hy) {
+    constexpr int ID = TOMO_MAX_NC - 1;
+    const int nc = S.nc, ny = S.ny;
+    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+    const bool hy_ok = (y >= 0 && y < ny);
+    const float dEx = lane_next(c.ez) - c.ez;
+    const float dEy = ez_up - c.ez;
+    c.phx = L.bxh * c.phx + L.axh * dEx;
+    c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+    const float ty = dEx * L.kxh + S.dxf * c.phx;
+    const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
+    const bool hok = L.hx_ok && hy_ok;
+    hy = hok ? c.hy + S.chy * ty : 0.f;
+    hx = hok ? c.hx - S.chx * tx : 0.f;
+}
+
+// E half-step of one row given its new H (and Hx of the row below in hxp,
+// which receives this row's Hx).
+__device__ __forceinline__ void res_e(const SimView& S, const float4* __restrict__ lc4, const float* __restrict__ lcp,
+                                      const ResLane& L, Cell& c, float hx, float hy, float& hxp, int y, int x, int n,
+                                      bool wsrc) {
+    constexpr int ID = TOMO_MAX_NC - 1;
+    const int nc = S.nc, ny = S.ny;
+    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+    const bool ey_ok = (y >= 1 && y <= ny - 1);
+    const float dHy = hy - lane_prev(hy);
+    const float dHx = hx - hxp;
+    hxp = hx;
+    c.pex = L.bxe * c.pex + L.axe * dHy;
+    c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+    const float cy = dHy * L.kxe + S.dxf * c.pex;
+    const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+    float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
+    if (wsrc && y == S.sy && x == S.sx) en += S.srcw[n];
+    const bool eok = L.ex_ok && ey_ok;
+    c.jp = eok ? qb.y * c.jp + qb.z * (en - c.ez) : c.jp;
+    c.ez = eok ? en : 0.f;
+    c.hx = hx;
+    c.hy = hy;
+}
+
+// Row state the resident kernel keeps in registers; the CPML psi of waves
+// that need it live in LDS.
+struct RCell {
+    float ez, hx, hy, jp;
+    int mat;
+};
+
+template <int T, int RW, int MODE>
+__global__ __launch_bounds__(kResWaves * kWave, 1) void k_tres(const ResTile* __restrict__ tiles,
+                                                              const LaunchItem* __restrict__ items,
+                                                              const DevSim* __restrict__ sims, int* done,
+                                                              int* err, int spin_limit) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcp[8 * TOMO_MAX_NC];
+    __shared__ float ezb[kResWaves][kWave];
+    __shared__ float hxt[kResWaves][kWave];
+    __shared__ float4 lpsi[kResWaves][RW][kWave];  // {pex, phx, pey, phy} of general waves
+    __shared__ int s_abort;
+    constexpr int ID = TOMO_MAX_NC - 1;
+    const int tid = blockIdx.x;
+    const ResTile tile = tiles[tid];
+    const LaunchItem it = items[tile.item];
+    const DevSim* Sp = sims + it.sim;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+    if constexpr (MODE == TOMO_MODE_PHYSICAL) {
+        fill_edge<T, false>(Sp, lc4, lcp, nullptr);
+    } else {
+        // the reference's tables as they are (its stage divides by kappa)
+        float* lcf = (float*)lc4;
+        for (int i = threadIdx.x; i < Sp->nmat * TOMO_NCOEF; i += blockDim.x) lcf[i] = Sp->coef[i];
+        for (int i = threadIdx.x; i < Sp->nc; i += blockDim.x) {
+            CP(0, i) = Sp->ke[i]; CP(1, i) = Sp->km[i]; CP(2, i) = Sp->ike[i]; CP(3, i) = Sp->ikm[i];
+            CP(4, i) = Sp->ae[i]; CP(5, i) = Sp->be[i]; CP(6, i) = Sp->am[i]; CP(7, i) = Sp->bm[i];
+        }
+    }
+    if (threadIdx.x == 0) s_abort = 0;
+    __syncthreads();
+    SimView S;
+    S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
+    S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
+    S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
+    S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
+    S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
+    S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
+    S.chx = Sp->chx; S.chy = Sp->chy; S.chx_d = Sp->chx_d; S.chy_d = Sp->chy_d; S.dxf = Sp->dxf; S.dyf = Sp->dyf;
+    S.inc_dt = Sp->inc_dt; S.inc_c = Sp->inc_c; S.inc_ax = Sp->inc_ax; S.inc_ay = Sp->inc_ay;
+    S.inc_itau = Sp->inc_itau; S.inc_abs = Sp->inc_abs_origin;
+    const int nx = S.nx, ny = S.ny, nc = S.nc, D = S.D;
+    const int x0 = tile.x0 - T;  // first lane's column
+    const int x = x0 + lane;
+    const bool xin = (x >= 0 && x <= nx);
+    const bool xown = xin && lane >= T && lane < kWave - T;
+    const bool gl = lane < T || lane >= kWave - T;          // ghost lane
+    const bool fl = lane < 2 * T || lane >= kWave - 2 * T;  // frame lane (or ghost)
+    WaveFlags wf;
+    wf.xs = (x0 <= nc) || (x0 + kWave - 1 >= nx - nc);
+    wf.ccol = (x0 <= D + 1 && x0 + kWave - 1 >= D) || (x0 <= nx - D && x0 + kWave - 1 >= nx - D - 1);
+    wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
+    ResLane L{};
+    if constexpr (MODE == TOMO_MODE_PHYSICAL) {
+        const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+        const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+        L.kxe = CP(2, ixe); L.axe = CP(4, ixe); L.bxe = CP(5, ixe);
+        L.kxh = CP(3, ixh); L.axh = CP(6, ixh); L.bxh = CP(7, ixh);
+        L.hx_ok = (x >= 0 && x < nx);
+        L.ex_ok = (x >= 1 && x <= nx - 1);
+    }
+    const int y0 = tile.y0, y1 = tile.y1;
+    const int yb = y0 - T + wv * RW;  // this wave's first row
+    // Clean waves (every row and lane clear of CPML strips, contour rows and
+    // columns, PEC and the grid's end, like an interior tile) take the bare
+    // Yee/Debye stage; the others the general one with psi in LDS.
+    const int cbot = max(nc, D + 1) + 1, ctop = min(ny - nc, ny - D - 1) - 1;
+    const int cl = max(nc, D + 1) + 1, cr = min(nx - nc, nx - D - 1) - 1;
+    const bool clean = MODE == TOMO_MODE_PHYSICAL && yb >= cbot && yb + RW - 1 <= ctop && x0 >= cl &&
+                       x0 + kWave - 1 <= cr;
+    const int nblocks = (it.nsteps + T - 1) / T;
+    RCell rows[RW];
+    float4* psi = lpsi[wv][0] + lane;  // row i at psi[i * kWave]
+    auto put = [&](int i, const Cell& c) {
+        rows[i].ez = c.ez; rows[i].hx = c.hx; rows[i].hy = c.hy; rows[i].jp = c.jp; rows[i].mat = c.mat;
+        if (!clean) psi[i * kWave] = make_float4(c.pex, c.phx, c.pey, c.phy);
+    };
+    auto get = [&](int i) {
+        Cell c;
+        c.ez = rows[i].ez; c.hx = rows[i].hx; c.hy = rows[i].hy; c.jp = rows[i].jp; c.mat = rows[i].mat;
+        const float4 q = clean ? make_float4(0.f, 0.f, 0.f, 0.f) : psi[i * kWave];
+        c.pex = q.x; c.phx = q.y; c.pey = q.z; c.phy = q.w;
+        return c;
+    };
+    S.par = it.parity0;
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+        // an opaque row index keeps the compiler from hoisting every row's
+        // addresses out of the block loop (which would exhaust the registers)
+        int y = __builtin_amdgcn_readfirstlane(yb + i);
+        asm volatile("" : "+s"(y));
+        Cell c;
+        load_cell(S, lc4, y, x, xin, wf, c);
+        put(i, c);
+    }
+    for (int b = 0; b < nblocks; ++b) {
+        S.par = it.parity0 ^ (b & 1);
+        if (b > 0) {
+            // wait until every neighbour has stored its frame of block b - 1
+            if (threadIdx.x == 0) {
+                int spins = 0, ok = 1;
+                for (int k = 0; k < 8 && ok; ++k) {
+                    const int nbt = tile.nb[k];
+                    if (nbt < 0) continue;
+                    while (__hip_atomic_load(done + nbt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < b) {
+                        if (++spins > spin_limit ||
+                            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                            ok = 0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                if (!ok) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_abort = 1;
+                }
+            }
+            __syncthreads();
+            if (s_abort) return;  // uniform over the workgroup
+            // ghost rows and ghost lanes from the neighbours' frames (coherent
+            // loads: the frames were written by other workgroups of this launch)
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                int y = __builtin_amdgcn_readfirstlane(yb + i);
+                asm volatile("" : "+s"(y));
+                if (y < y0 || y >= y1 || gl) {
+                    Cell c;
+                    load_cell<true>(S, lc4, y, x, xin, wf, c);
+                    put(i, c);
+                }
+            }
+        }
+        const int step0 = it.step0 + b * T, k0 = b * T;
+        const int nst = min(T, it.nsteps - b * T);
+        for (int t = 0; t < nst; ++t) {
+            const int n = step0 + t;
+            ezb[wv][lane] = rows[0].ez;
+            __syncthreads();
+            const float ez_top = wv + 1 < kResWaves ? ezb[wv + 1][lane] : 0.f;
+            // H half-step of the top row first: the wave above needs its Hx
+            float htx, hty;
+            if (clean) {
+                const RCell& tp = rows[RW - 1];
+                const float dEx = lane_next(tp.ez) - tp.ez;
+                const float dEy = ez_top - tp.ez;
+                hty = tp.hy + S.chy * dEx;
+                htx = tp.hx - S.chx * dEy;
+            } else {
+                int yt = __builtin_amdgcn_readfirstlane(yb + RW - 1);
+                asm volatile("" : "+s"(yt));
+                Cell tc = get(RW - 1);
+                if constexpr (MODE == TOMO_MODE_PHYSICAL) res_h(S, lcp, L, tc, ez_top, yt, htx, hty);
+                else stage_h<MODE>(S, lcp, wf, tc, ez_top, x, yt, htx, hty);
+                psi[(RW - 1) * kWave] = make_float4(tc.pex, tc.phx, tc.pey, tc.phy);
+            }
+            hxt[wv][lane] = htx;
+            __syncthreads();
+            float hxp = wv > 0 ? hxt[wv - 1][lane] : 0.f;
+            if (clean) {
+                const float chx = S.chx, chy = S.chy;
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    RCell& c = rows[i];
+                    float hx = htx, hy = hty;
+                    if (i + 1 < RW) {
+                        const float dEx = lane_next(c.ez) - c.ez;
+                        const float dEy = rows[i + 1].ez - c.ez;
+                        hy = c.hy + chy * dEx;
+                        hx = c.hx - chx * dEy;
+                    }
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp;
+                    hxp = hx;
+                    // opaque per step: the coefficient reads stay in the step
+                    // loop instead of occupying registers for the whole kernel
+                    int m = c.mat;
+                    asm volatile("" : "+v"(m));
+                    const float4 qa = lc4[2 * m], qb = lc4[2 * m + 1];
+                    float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
+                    if (wf.src && yb + i == S.sy && x == S.sx) en += S.srcw[n];
+                    c.jp = qb.y * c.jp + qb.z * (en - c.ez);
+                    c.ez = en;
+                    c.hx = hx;
+                    c.hy = hy;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    // opaque per step: the row's CPML and material reads stay in
+                    // the step loop instead of occupying registers for the kernel
+                    int y = __builtin_amdgcn_readfirstlane(yb + i);
+                    asm volatile("" : "+s"(y));
+                    Cell c = get(i);
+                    asm volatile("" : "+v"(c.mat));
+                    float hx = htx, hy = hty;
+                    if constexpr (MODE == TOMO_MODE_PHYSICAL) {
+                        if (i + 1 < RW) res_h(S, lcp, L, c, rows[i + 1].ez, y, hx, hy);
+                        res_e(S, lc4, lcp, L, c, hx, hy, hxp, y, x, n, wf.src);
+                    } else {
+                        if (i + 1 < RW) stage_h<MODE>(S, lcp, wf, c, rows[i + 1].ez, x, y, hx, hy);
+                        stage_e<MODE>(S, lc4, lcp, wf, c, hx, hy, hxp, x, y, n);
+                    }
+                    put(i, c);
+                    if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
+                }
+            }
+        }
+        // the frame (or, after the last block, every owned cell) to plane par ^ 1
+        const bool last = b + 1 == nblocks;
+        if (last) {
+            // the state for later launches: plain stores (the launch boundary orders them)
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                int y = __builtin_amdgcn_readfirstlane(yb + i);
+                asm volatile("" : "+s"(y));
+                if (y >= y0 && y < y1 && xown) store_cell(S, lc4, y, x, wf, get(i));
+            }
+        } else {
+            // the frame, with coherent stores; once they have all completed
+            // (vmcnt 0 in every wave, then the barrier) the counter publishes it
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                int y = __builtin_amdgcn_readfirstlane(yb + i);
+                asm volatile("" : "+s"(y));
+                if (y >= y0 && y < y1 && xown && (fl || y < y0 + T || y >= y1 - T))
+                    store_cell<true>(S, lc4, y, x, wf, get(i));
+            }
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(done + tid, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
 #undef CP
 
 // Row-edge tiles of physical-mode simulations: the top and bottom rows
@@ -1810,6 +2163,42 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
     TF(1) TF(2) TF(4) TF(6) TF(8)
 #undef TF
     return hipErrorInvalidValue;
+}
+
+bool tres_supported(int T, int rows_per_wave) {
+    return (T == 2 || T == 4 || T == 8) && (rows_per_wave == 8 || rows_per_wave == 12);
+}
+
+template <int T, int RW>
+static const void* tres_fn(int mode) {
+    return mode == TOMO_MODE_PHYSICAL ? reinterpret_cast<const void*>(&k_tres<T, RW, TOMO_MODE_PHYSICAL>)
+                                      : reinterpret_cast<const void*>(&k_tres<T, RW, TOMO_MODE_REF_COMPAT>);
+}
+
+static const void* tres_kernel(int T, int RW, int mode) {
+#define TR(TT, RR) \
+    if (T == TT && RW == RR) return tres_fn<TT, RR>(mode);
+    TR(2, 8) TR(2, 12) TR(4, 8) TR(4, 12) TR(8, 8) TR(8, 12)
+#undef TR
+    return nullptr;
+}
+
+int tres_max_tiles(int T, int rows_per_wave, int mode, int cus) {
+    const void* f = tres_kernel(T, rows_per_wave, mode);
+    if (!f) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kResWaves * kWave, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+hipError_t launch_tres(int T, int rows_per_wave, int mode, const ResTile* tiles, int ntiles,
+                       const LaunchItem* items, const DevSim* sims, int* done, int* err, int spin_limit,
+                       hipStream_t st) {
+    const void* f = tres_kernel(T, rows_per_wave, mode);
+    if (!f) return hipErrorInvalidValue;
+    t_start = t_stop = nullptr;  // a cooperative launch carries no events
+    void* args[] = {(void*)&tiles, (void*)&items, (void*)&sims, (void*)&done, (void*)&err, (void*)&spin_limit};
+    return hipLaunchCooperativeKernel(f, dim3(ntiles), dim3(kResWaves * kWave), args, 0, st);
 }
 
 hipError_t launch_project_mfma(const int* sim_idx, int n_sims, int n_freq, const DevSim* sims, const float2* kinv,

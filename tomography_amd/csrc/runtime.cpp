@@ -126,6 +126,12 @@ struct Staging {
     int ntiles = 0, n_edge = 0;  // tiles: n_edge edge tiles first, then interior tiles
     int n_rowedge = 0;           // edge tiles: n_corner general ones, then row-edge, then column-edge
     int n_corner = 0;
+    // resident form: tiles, per-tile completed-block counters and the error word
+    ResTile* h_res = nullptr;
+    ResTile* d_res = nullptr;
+    int res_cap = 0, nres = 0;
+    int* d_flags = nullptr;  // res_cap counters, then the error word (res_err)
+    int res_key_form = 0;    // form the tile lists were built for
 };
 }  // namespace
 
@@ -211,6 +217,26 @@ int ensure_table(tomo_ctx* c, int need) {
     for (int i = ncap - 1; i >= c->cap; --i) c->free_idx.push_back(i);
     c->d_sims = nd;
     c->cap = ncap;
+    return TOMO_OK;
+}
+
+int ensure_res_staging(tomo_ctx* c, Staging& st, int nres) {
+    if (nres <= st.res_cap) return TOMO_OK;
+    if (st.fixed) return fail(TOMO_E_NOMEM, "resident tiles exceed the fixed staging");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st.h_res) HIPCHK(hipHostFree(st.h_res));
+    if (st.d_res) HIPCHK(hipFree(st.d_res));
+    if (st.d_flags) HIPCHK(hipFree(st.d_flags));
+    const int cap = std::max(nres, 2 * st.res_cap);
+    st.h_res = nullptr;
+    st.d_res = nullptr;
+    st.d_flags = nullptr;
+    st.res_cap = 0;
+    HIPCHK(hipHostMalloc(&st.h_res, sizeof(ResTile) * cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&st.d_res, sizeof(ResTile) * cap));
+    HIPCHK(hipMalloc(&st.d_flags, sizeof(int) * (cap + 64)));
+    st.res_cap = cap;
+    st.tile_key.clear();
     return TOMO_OK;
 }
 
@@ -524,6 +550,40 @@ void build_tiles(Slot* sl, int item, int T, int S, int Si, int W, int Wi, int ER
     edge_rows_split(ib, ny + 1);
 }
 
+// Tiles of the resident form for one simulation: bands of 64 - 2T owned
+// columns, row blocks of at most 8 RW - 2T owned rows (balanced), each with
+// its eight neighbours' indices in `out`.
+void build_res_tiles(const Slot* sl, int item, int T, int RW, std::vector<ResTile>& out) {
+    const DevSim& d = sl->d;
+    const int V = 64 - 2 * T, R = kResidentWaves * RW - 2 * T;
+    const int nbx = (d.nx + 1 + V - 1) / V, nby = (d.ny + 1 + R - 1) / R;
+    const int rb = (d.ny + 1 + nby - 1) / nby;
+    const int base = (int)out.size();
+    for (int ry = 0; ry < nby; ++ry)
+        for (int bx = 0; bx < nbx; ++bx) {
+            ResTile t{};
+            t.item = item;
+            t.x0 = bx * V;
+            t.y0 = ry * rb;
+            t.y1 = std::min(t.y0 + rb, d.ny + 1);
+            int k = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (!dx && !dy) continue;
+                    const int X = bx + dx, Y = ry + dy;
+                    t.nb[k++] = (X >= 0 && X < nbx && Y >= 0 && Y < nby) ? base + Y * nbx + X : -1;
+                }
+            out.push_back(t);
+        }
+}
+
+// The resident launches' error word: 16-byte aligned past the counters.
+int* res_err(const Staging& st) { return st.d_flags + ((st.res_cap + 3) & ~3) + 32; }
+
+// Polls a resident workgroup makes before it gives up on a neighbour (about
+// a second of waiting at the kernel's poll interval).
+constexpr int kResSpinLimit = 1 << 24;
+
 hipEvent_t pool_event(tomo_ctx* c) {
     if (!c->ev_pool.empty()) {
         hipEvent_t e = c->ev_pool.back();
@@ -597,6 +657,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
         return fail(TOMO_E_INVALID, "unsupported schedule");
     for (Slot* s : sl)
         if (s->d.chunk != K) return fail(TOMO_E_STATE, "simulation was created under a different DFT chunk");
+    bool res_used = false;  // a resident launch ran: its error word is checked at the end
     for (int mode = 0; mode < 2; ++mode) {
         std::vector<Slot*> ms;
         for (Slot* s : sl)
@@ -609,10 +670,19 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
             for (Slot* s : ms) key.push_back(s->serial);
             std::vector<TileItem> te, tr, tc, tn;
             const bool rebuild = key != st.tile_key || st.key_T != T || st.key_S != S || st.key_W != W ||
-                                 st.key_Wi != c->Wi || st.key_ER != c->ER || st.key_Si != c->Si;
+                                 st.key_Wi != c->Wi || st.key_ER != c->ER || st.key_Si != c->Si ||
+                                 st.res_key_form != c->form;
+            // resident form: every tile of the set in one cooperative launch,
+            // when they fit on the device at once (else the fused form runs)
+            const bool want_res = c->form == TOMO_FORM_RESIDENT && tres_supported(T, S) && !st.fixed;
+            std::vector<ResTile> trs;
             if (rebuild) {
                 for (size_t i = 0; i < ms.size(); ++i)
                     build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
+                if (want_res) {
+                    for (size_t i = 0; i < ms.size(); ++i) build_res_tiles(ms[i], (int)i, T, S, trs);
+                    if ((int)trs.size() > tres_max_tiles(T, S, mode, c->cus)) trs.clear();
+                }
             }
             const int ntiles = rebuild ? (int)(te.size() + tr.size() + tc.size() + tn.size()) : st.ntiles;
             // Items of E consecutive epochs in one upload: every simulation of
@@ -658,7 +728,19 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                 st.n_edge = (int)(te.size() + tr.size() + tc.size());
                 st.n_corner = (int)te.size();
                 st.n_rowedge = (int)tr.size();
+                st.res_key_form = c->form;
+                st.nres = 0;
+                if (!trs.empty()) {
+                    rc = ensure_res_staging(c, st, (int)trs.size());
+                    if (rc) return rc;
+                    std::copy(trs.begin(), trs.end(), st.h_res);
+                    HIPCHK(hipMemcpyAsync(st.d_res, st.h_res, sizeof(ResTile) * trs.size(), hipMemcpyHostToDevice,
+                                          c->stream));
+                    st.nres = (int)trs.size();
+                    st.tile_key = key;  // ensure_res_staging may have cleared it
+                }
             }
+            const bool resident = want_res && st.nres > 0;
             if (st.staged) HIPCHK(hipEventRecord(st.staged, c->stream));
             const int n_edge = st.n_edge, n_inner = st.ntiles - st.n_edge;
             const int cap = c->cus * c->grid_mult;
@@ -671,8 +753,9 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
             const int cnt[3] = {n0, nr, ncl};
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
-            const bool fused = mode == TOMO_MODE_PHYSICAL && c->form == TOMO_FORM_FUSED;
-            const bool multi = mode == TOMO_MODE_PHYSICAL && !fused && n_inner > 0 && nside > 0;
+            const bool fused = mode == TOMO_MODE_PHYSICAL && !resident &&
+                               (c->form == TOMO_FORM_FUSED || c->form == TOMO_FORM_RESIDENT);
+            const bool multi = mode == TOMO_MODE_PHYSICAL && !fused && !resident && n_inner > 0 && nside > 0;
             // per-class cell-steps and bytes of block o (kernel timing only)
             auto block_work = [&](int o, int k, double& cells, double& bytes) {
                 cells = bytes = 0;
@@ -689,14 +772,36 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                 if (multi) HIPCHK(hipEventRecord(c->ev_bar, c->stream));
                 double ucells = 0, ubytes = 0;
                 hipEvent_t first = nullptr;  // start of the update phase: its first launch
-                for (int o = 0; o < nl; ++o) {
+                if (resident) {
+                    // every time block of the epoch in one launch; the counters start at zero
+                    for (int o = 0; o < nl; ++o)
+                        for (int k = 0; k < 4; ++k) {
+                            double cl = 0, by = 0;
+                            block_work(o, k, cl, by);
+                            ucells += cl;
+                            ubytes += by;
+                        }
+                    if (!res_used) HIPCHK(launch_zero(res_err(st), 16, c->stream));
+                    res_used = true;
+                    HIPCHK(launch_zero(st.d_flags, (sizeof(int) * st.nres + 15) & ~(size_t)15, c->stream));
+                    hipEvent_t a = c->prof ? pool_event(c) : nullptr, b = c->prof ? pool_event(c) : nullptr;
+                    if (a && b) HIPCHK(hipEventRecord(a, c->stream));
+                    HIPCHK(launch_tres(T, S, mode, st.d_res, st.nres, d_it, st.d_sims, st.d_flags,
+                                       res_err(st), kResSpinLimit, c->stream));
+                    if (a && b) {
+                        HIPCHK(hipEventRecord(b, c->stream));
+                        c->pending.push_back({a, b, TOMO_KERNEL_RESIDENT, ucells, ubytes, true});
+                        first = a;
+                    }
+                }
+                for (int o = 0; !resident && o < nl; ++o) {
                     double cells[4], bytes[4];
                     for (int k = 0; k < 4; ++k) {
                         block_work(o, k, cells[k], bytes[k]);
                         ucells += cells[k];
                         ubytes += bytes[k];
                     }
-                    if (mode != TOMO_MODE_PHYSICAL) {
+                    if (mode != TOMO_MODE_PHYSICAL) {  // (the resident form skipped this loop)
                         LaunchTimer lt(c);
                         HIPCHK(launch_tblock(T, mode, st.d_tiles, n_edge, d_it, st.d_sims, o, W,
                                              std::min(n_edge, cap), c->stream));
@@ -768,7 +873,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                     minn = std::min(minn, h_it[i].nsteps);
                 }
                 nl = (maxn + T - 1) / T;
-                if (c->use_graph && !st.fixed && !c->prof && minn == K) {
+                if (c->use_graph && !st.fixed && !resident && !c->prof && minn == K) {
                     const std::vector<int64_t> gk = {mode, T, S, c->Si, W, c->Wi, c->ER, c->form, K, c->grid_mult, ntiles, maxP,
                                                      fp32_any, fp64_any, (int64_t)ms.size(), (int64_t)(intptr_t)d_it,
                                                      (int64_t)(intptr_t)st.d_tiles, (int64_t)(intptr_t)st.d_sims};
@@ -813,6 +918,12 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                 if (s->done < s->target) next.push_back(s);
             ms.swap(next);
         }
+    }
+    if (res_used) {
+        int err = 0;
+        HIPCHK(hipMemcpyAsync(&err, res_err(st), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (err) return fail(TOMO_E_HIP, "resident update: a workgroup gave up waiting for its neighbours");
     }
     return TOMO_OK;
 }
@@ -1372,6 +1483,9 @@ int tomo_ctx_destroy(tomo_ctx* c) {
     if (c->stg.h_tiles) (void)hipHostFree(c->stg.h_tiles);
     if (c->stg.d_tiles) (void)hipFree(c->stg.d_tiles);
     if (c->stg.staged) (void)hipEventDestroy(c->stg.staged);
+    if (c->stg.h_res) (void)hipHostFree(c->stg.h_res);
+    if (c->stg.d_res) (void)hipFree(c->stg.d_res);
+    if (c->stg.d_flags) (void)hipFree(c->stg.d_flags);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     (void)drain_timings(c);
     release_cache(c);
@@ -1394,7 +1508,7 @@ static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
     const int ER = s.edge_rows ? s.edge_rows : c->ER;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
     if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64 ||
-        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED) || ER < 1 || ER > 4096 ||
+        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED && F != TOMO_FORM_RESIDENT) || ER < 1 || ER > 4096 ||
         (s.inner_rows != 0 && (s.inner_rows < 4 || s.inner_rows > 4096)))
         return fail(TOMO_E_INVALID, "bad schedule");
     c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F; c->ER = ER;

@@ -99,3 +99,16 @@ struct TileItem {
     int pad;
 };
 
+
+// Tile of the resident update form: one workgroup keeps a band of 64 lanes
+// (owned columns [x0, x0 + 64 - 2T), T ghost columns on each side) times the
+// owned rows [y0, y1) plus T ghost rows above and below in registers for a
+// whole epoch, and exchanges only a T-deep frame with its neighbours between
+// time blocks.  nb: the eight neighbouring tiles of the same simulation
+// (-1 where the grid ends), whose completed-block counters it waits for.
+struct ResTile {
+    int item;      // index into the LaunchItem array
+    int x0;        // first owned column
+    int y0, y1;    // owned rows
+    int nb[8];
+};

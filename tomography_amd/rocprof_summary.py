@@ -6,6 +6,9 @@
         per-kernel, per-template average of COUNTER (KiB per dispatch) from
         PREFIX_counter_collection.csv, plus the copy-kernel dispatches used
         for calibration
+    python -m tomography_amd.rocprof_summary db FILE.db [KERNEL=LAST ...]
+        the same per-kernel figures as `trace` from rocprofv3's default
+        SQLite output (PREFIX_results.db)
 
 Kernel names are shortened to `k_name<T>` (the fused step count) so the two
 bench workloads, which run different T, stay apart.
@@ -95,11 +98,39 @@ def pmc(d, prefix, counter, spec=()):
             "largest_copy_dispatches": sorted(copies, key=lambda c: -c["kib"])[:3]}
 
 
+def db(path, spec=()):
+    """Per-kernel dispatch statistics from a rocprofv3 results database."""
+    import sqlite3
+    last = _last(spec)
+    con = sqlite3.connect(path)
+    per = defaultdict(list)
+    info = {}
+    q = ("select name, dispatch_id, duration, vgpr_count, accum_vgpr_count, sgpr_count, lds_size, scratch_size, "
+         "workgroup_x, grid_x from kernels")
+    for name, did, dur, vg, ag, sg, lds, scr, wg, grid in con.execute(q):
+        n = short_name(name)
+        per[n].append((int(did), int(dur)))
+        info[n] = {"vgpr": vg, "agpr": ag, "sgpr": sg, "lds": lds, "scratch": scr, "wg": wg, "grid": grid}
+    out = {}
+    for n, v in sorted(per.items(), key=lambda kv: -sum(x[1] for x in kv[1])):
+        v.sort()
+        rec = {"calls": len(v), "total_ms": sum(x[1] for x in v) / 1e6, "avg_us": sum(x[1] for x in v) / len(v) / 1e3}
+        if n in last:
+            w = v[-last[n]:]
+            rec["last"] = len(w)
+            rec["avg_us_last"] = sum(x[1] for x in w) / len(w) / 1e3
+        rec.update(info[n])
+        out[n] = rec
+    return out
+
+
 def main(argv):
     if len(argv) >= 3 and argv[0] == "stats":
         print(json.dumps(stats(argv[1], argv[2]), indent=1))
     elif len(argv) >= 3 and argv[0] == "trace":
         print(json.dumps(trace(argv[1], argv[2], argv[3:]), indent=1))
+    elif len(argv) >= 2 and argv[0] == "db":
+        print(json.dumps(db(argv[1], argv[2:]), indent=1))
     elif len(argv) >= 4 and argv[0] == "pmc":
         print(json.dumps(pmc(argv[1], argv[2], argv[3], argv[4:]), indent=1))
     else:
