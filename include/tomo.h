@@ -149,7 +149,8 @@ const char *tomo_last_error(void);
  * wave, 8 waves) in registers through every time block, exchanging only a
  * T-deep frame with its neighbours; seg_rows 8 or 12, tsteps 2, 4 or 8.  Runs
  * that hold more tiles than the device can keep co-resident use the fused
- * form instead. */
+ * form instead, and so does a context the device refuses a cooperative launch
+ * (tomo_stats.resident_refused). */
 #define TOMO_FORM_RESIDENT 3
 typedef struct tomo_schedule {
     int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows;
@@ -159,6 +160,12 @@ typedef struct tomo_schedule {
 } tomo_schedule;
 int tomo_set_schedule(tomo_ctx *ctx, const tomo_schedule *s);
 int tomo_get_schedule(tomo_ctx *ctx, tomo_schedule *s);
+/* The schedule the FDTD_GPU / FDTD_CPU shims install on their context: the
+ * fastest measured at the drop-in size (401^2, 4 excitations advancing
+ * together): resident tiles of 12 rows, 8 steps per frame exchange, 64-step
+ * DFT chunks.  The DFT chunk sets the FP32 partial sums of the running DFT,
+ * so contexts compare bit for bit only under equal chunks. */
+#define TOMO_DROPIN_SCHEDULE {8, 12, 1, 1, 64, 16, TOMO_FORM_RESIDENT, 1, 0}
 
 /* Autotuner: time candidate schedules on n_sims resident simulations of
  * `cfg` (coordinate descent over form, tsteps, seg_rows, inner_rows,
@@ -199,6 +206,10 @@ typedef struct tomo_stats {
     /* request records the context holds: queued, resident, or finished or
      * cancelled and not yet collected by tomo_poll / tomo_wait */
     int64_t requests;
+    /* 1 once the device has refused this context a cooperative launch (one
+     * process at a time may hold a device's cooperative queue): its
+     * TOMO_FORM_RESIDENT epochs then run the fused form, with the same results */
+    int64_t resident_refused;
 } tomo_stats;
 int tomo_profile(tomo_ctx *ctx, int enable); /* (re)starts the counters */
 /*

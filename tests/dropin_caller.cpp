@@ -6,7 +6,8 @@
 //
 // It checks, and prints one line per check:
 //   * FDTD_GPU on the 27-parameter bounds of test.cxx:55-77 returns the same
-//     double as tomo_fitness(REF_GPU) for the same vector, and is <= 0;
+//     double as tomo_fitness(REF_GPU) for the same vector on a context under
+//     the shims' schedule (TOMO_DROPIN_SCHEDULE), and is <= 0;
 //   * FDTD_CPU on a 162-parameter vector returns tomo_fitness(REF_CPU);
 //   * a vector shorter than the path reads throws std::out_of_range, as the
 //     reference's vector::at does (FDTD_GPU.cu:655-693, FDTD_CPU.cxx:681-698);
@@ -42,6 +43,7 @@ static void check(bool ok, const char* what, double a, double b) {
 }
 
 int main() {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);  // each check line reaches a pipe as it is printed
     // test.cxx:55-77: 9 eps_inf in [1, 7], 9 delta_eps in [1.5, 41], 9 sigma in [0, 0]
     vector<vector<double>> pop;
     for (int k = 0; k < 3; ++k) {
@@ -55,6 +57,12 @@ int main() {
     tomo_ctx* ctx = nullptr;
     if (tomo_ctx_create(0, 0, &ctx) != TOMO_OK) {
         std::printf("FAIL tomo_ctx_create: %s\n", tomo_last_error());
+        return 1;
+    }
+    // the shims' schedule: the DFT chunk sets the running DFT's partial sums
+    const tomo_schedule sched = TOMO_DROPIN_SCHEDULE;
+    if (tomo_set_schedule(ctx, &sched) != TOMO_OK) {
+        std::printf("FAIL tomo_set_schedule: %s\n", tomo_last_error());
         return 1;
     }
     for (size_t k = 0; k < pop.size(); ++k) {
@@ -83,6 +91,9 @@ int main() {
         check(threw, which ? "FDTD_CPU(short vector) throws std::out_of_range"
                            : "FDTD_GPU(short vector) throws std::out_of_range", threw, 1);
     }
+    tomo_stats st{};
+    if (tomo_get_stats(ctx, &st) == TOMO_OK)
+        std::printf("INFO resident_refused %lld\n", (long long)st.resident_refused);
     tomo_ctx_destroy(ctx);
     std::printf("%s\n", fails ? "FAILED" : "ALL PASS");
     return fails ? 1 : 0;

@@ -102,6 +102,14 @@ def test_header_constants_and_struct_layouts_match_the_python_mirror(tmp_path):
     assert got == [ctypes.sizeof(t.SimConfig), ctypes.sizeof(t.api.Schedule), ctypes.sizeof(t.api.Stats),
                    ctypes.sizeof(t.api.KernelStats), ctypes.sizeof(t.api.SearchConfig),
                    ctypes.sizeof(t.api.SearchResult)]
+    # the shims' schedule is the one the drop-in benchmark and tests set
+    from tomography_amd.workloads import REF_SCHEDULE, SCHED_KEYS
+    src.write_text('#include <stdio.h>\n#include "tomo.h"\nint main(void){tomo_schedule s = TOMO_DROPIN_SCHEDULE;'
+                   'printf("%d %d %d %d %d %d %d %d %d\\n", s.tsteps, s.seg_rows, s.edge_waves, s.inner_waves,'
+                   's.chunk, s.grid_mult, s.form, s.edge_rows, s.inner_rows);return 0;}\n')
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True).stdout.split()]
+    assert got == [REF_SCHEDULE[k] for k in SCHED_KEYS]
 
 
 def test_oracle_and_host_setup_clean_under_address_and_ub_sanitizers():

@@ -7,15 +7,19 @@
   * the phase-aware fitness (TOMO_FLAG_PHASE_FITNESS, FDTD_GPU.cu:1009-1018):
     the GPU reduction against the oracle's arithmetic on the same records;
   * the asynchronous search driver's fitness values against the oracle;
-  * a C++ program that calls FDTD_GPU / FDTD_CPU like the reference's driver.
+  * a C++ program that calls FDTD_GPU / FDTD_CPU like the reference's driver;
+  * the resident form in a second process, which the device refuses its
+    cooperative queue: the epochs run fused, with the same results.
 
 Tolerances: far fields and DFT accumulators <= 1e-4 relative to FP64; the
 fitness reduction of given records within 1e-12 (both are FP64 sums of the
 same terms, in different orders).
 """
 import ctypes as C
+import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -181,7 +185,10 @@ def test_cxx_dropin_caller():
     """tests/dropin_caller.cpp, linked against libtomo.so exactly as the
     reference's driver links FDTD_GPU.cu (test.cxx:94-121): FDTD_GPU and
     FDTD_CPU return tomo_fitness's value, are callable through function
-    pointers, and a short vector throws std::out_of_range."""
+    pointers, and a short vector throws std::out_of_range.  Run after this
+    process's own resident launches, the child is refused the device's
+    cooperative queue (one process at a time holds it) and its resident
+    epochs run the fused form, with the same results."""
     exe = os.path.join(ROOT, "tests", "_bin", "dropin_caller")
     assert os.path.exists(exe), "build first: python -m tomography_amd.build"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
@@ -191,3 +198,46 @@ def test_cxx_dropin_caller():
     # the value printed by the shim is the Python objective entry's value too
     lines = [l for l in r.stdout.splitlines() if l.startswith("PASS FDTD_GPU == tomo_fitness")]
     assert len(lines) == 3
+
+
+_CHILD = """
+import json, sys
+sys.path.insert(0, {tests!r})
+import tomography_amd as t
+from cases import PHYS_PARAMS, phys_small
+cfg = phys_small(steps=256, n=128, n_exc=2)
+with t.Context() as c:
+    c.set_schedule(4, 8, 1, 64, 1, 16, t.FORM_RESIDENT)
+    c.target_farfield(cfg)
+    c.profile(True)
+    f = c.eval(cfg, PHYS_PARAMS)
+    st = c.stats()
+print(json.dumps(dict(fitness=f.hex(), refused=st["resident_refused"], tres=st["k_tres"]["launches"],
+                      fused=st["k_tfused"]["launches"])))
+"""
+
+
+def test_resident_form_refused_in_a_second_process_runs_fused():
+    """One process at a time may hold a device's cooperative queue.  This
+    process runs a resident evaluation (taking the queue); a child process
+    then evaluates the same vector under the same schedule.  A refused child
+    (tomo_stats.resident_refused) runs its epochs in the fused form, and its
+    fitness is the same double; a child that is not refused runs k_tres."""
+    cfg = phys_small(steps=256, n=128, n_exc=2)
+    with t.Context() as c:
+        c.set_schedule(4, 8, 1, 64, 1, 16, t.FORM_RESIDENT)
+        c.target_farfield(cfg)
+        c.profile(True)
+        want = c.eval(cfg, PHYS_PARAMS)
+        st = c.stats()
+        assert st["k_tres"]["launches"] > 0 and st["resident_refused"] == 0
+        code = _CHILD.format(tests=os.path.dirname(os.path.abspath(__file__)))
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    print(got)
+    assert float.fromhex(got["fitness"]) == want
+    if got["refused"]:
+        assert got["tres"] == 0 and got["fused"] > 0
+    else:
+        assert got["tres"] > 0

@@ -89,7 +89,7 @@ def test_phys_fields_vs_oracle(ctx, oracle):
 @pytest.mark.parametrize("sched", [(1, 32, 4, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 3, 48), (8, 64, 2, 64),
                                    (2, 8, 2, 32, 4, 2, t.FORM_FUSED), (8, 5, 4, 64, 1, 1, t.FORM_FUSED),
                                    (4, 16, 1, 32, 3, 8, t.FORM_FUSED), (8, 8, 4, 64, 4, 16, t.FORM_FUSED, 2),
-                                   (4, 32, 1, 64, 4, 16, t.FORM_STREAMS, 3),
+                                   (4, 32, 1, 64, 4, 16, t.FORM_STREAMS, 3), (6, 48, 1, 60, 4, 16, t.FORM_STREAMS, 24),
                                    (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 8), (2, 5, 2, 32, 4, 4, t.FORM_STREAMS, 2, 12),
                                    (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 14),
                                    (4, 8, 1, 32, 1, 16, t.FORM_RESIDENT), (8, 12, 1, 64, 1, 16, t.FORM_RESIDENT),

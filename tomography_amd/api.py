@@ -58,12 +58,14 @@ class Stats(_c.Structure):
     """tomo_stats (include/tomo.h): one record per kernel (TOMO_KERNEL_*) and
     one for the whole update phase."""
 
-    _fields_ = [("kernel", KernelStats * len(KERNEL_NAMES)), ("update", KernelStats), ("requests", _c.c_int64)]
+    _fields_ = [("kernel", KernelStats * len(KERNEL_NAMES)), ("update", KernelStats), ("requests", _c.c_int64),
+                ("resident_refused", _c.c_int64)]
 
     def as_dict(self):
         d = {name: self.kernel[i].as_dict() for i, name in enumerate(KERNEL_NAMES)}
         d["update"] = self.update.as_dict()
         d["requests"] = self.requests
+        d["resident_refused"] = self.resident_refused
         return d
 
 

@@ -64,6 +64,11 @@ tomo_ctx* ctx() {
         const char* d = std::getenv("TOMO_DEVICE");
         if (tomo_ctx_create(d ? std::atoi(d) : 0, 0, &g_ctx) != TOMO_OK)
             throw std::runtime_error(std::string("tomo_ctx_create: ") + tomo_last_error());
+        // the schedule measured fastest at the drop-in size (a grid whose
+        // tiles cannot all be co-resident runs the fused form by itself)
+        const tomo_schedule s = TOMO_DROPIN_SCHEDULE;
+        if (tomo_set_schedule(g_ctx, &s) != TOMO_OK)
+            throw std::runtime_error(std::string("tomo_set_schedule: ") + tomo_last_error());
     }
     return g_ctx;
 }

@@ -1401,6 +1401,18 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
     const int y0 = tile.y0, y1 = tile.y1;
     const int step0 = it.step0 + ordinal * T;
     const int k0 = ordinal * T;
+    // Sample slots of a contour lane, as lane constants: the first value
+    // (H on the face cells x = nx - D - 1 and x = D, Ez on their outer
+    // neighbours) goes to samp[(sk + k) P + sb + sd y] with sk = 0 or 2K; the
+    // face cells also store Ez K rows of P further on.  Leg xp runs up the
+    // column from cy_lo, leg xn down from ny - D - 1.
+    const bool s_face = (x == nx - D - 1 || x == D);
+    const bool s_ez = !s_face;
+    const int sd = (x >= nx - D - 1) ? 1 : -1;
+    const int sb = (x >= nx - D - 1) ? L1 - cy_lo : L3 + ny - D - 1;
+    const int s_off = (s_face ? 0 : 2 * K) * P + sb + k0 * P;
+    const bool s_lane = ccol && own;
+    const int ylo = max(max(y0, cy_lo), 0), yhi = min(y1, ny - D);  // rows of the contour's x legs
     const int w2 = 2 * nc;
     auto load = [&](int r, XCell& c) {
         c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
@@ -1474,21 +1486,12 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
         if (wcc) {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
-                const int y = r - 1 - t;
-                if ((FAST || t < nst) && ccol && own && y >= y0 && y < y1 && y >= cy_lo && y <= ny - D - 1) {
+                const int y = r - 1 - t;  // wave-uniform: the row test is a scalar branch
+                if ((FAST || t < nst) && y >= ylo && y < yhi && s_lane) {
                     const XCell& o = t + 1 < T ? cur[t + 1] : in;
-                    const int k = k0 + t;
-                    if (x == nx - D - 1) {
-                        samp[k * P + L1 + y - cy_lo] = o.hy;
-                        samp[(K + k) * P + L1 + y - cy_lo] = o.ez;
-                    } else if (x == nx - D) {
-                        samp[(2 * K + k) * P + L1 + y - cy_lo] = o.ez;
-                    } else if (x == D) {
-                        samp[k * P + L3 + (ny - D - 1 - y)] = o.hy;
-                        samp[(K + k) * P + L3 + (ny - D - 1 - y)] = o.ez;
-                    } else {
-                        samp[(2 * K + k) * P + L3 + (ny - D - 1 - y)] = o.ez;
-                    }
+                    gfloat* dst = samp + (s_off + t * P + sd * y);
+                    dst[0] = s_ez ? o.ez : o.hy;
+                    if (s_face) dst[(size_t)K * P] = o.ez;
                 }
             }
         }

@@ -158,6 +158,7 @@ struct tomo_ctx {
     int ER = 8;                        // output rows per corner / row-edge tile
     int Si = 0;                        // output rows per interior tile (0: S)
     bool use_graph = false;            // replay regular epochs from a captured hipGraph
+    bool res_refused = false;          // the device refused a cooperative launch: resident epochs run fused
     hipGraphExec_t gexec = nullptr;    // the captured epoch and what it was captured for
     std::vector<uint64_t> gkey_serials;
     std::vector<int64_t> gkey;
@@ -740,7 +741,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                     st.tile_key = key;  // ensure_res_staging may have cleared it
                 }
             }
-            const bool resident = want_res && st.nres > 0;
+            bool resident = want_res && st.nres > 0 && !c->res_refused;
             if (st.staged) HIPCHK(hipEventRecord(st.staged, c->stream));
             const int n_edge = st.n_edge, n_inner = st.ntiles - st.n_edge;
             const int cap = c->cus * c->grid_mult;
@@ -753,9 +754,9 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
             const int cnt[3] = {n0, nr, ncl};
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
-            const bool fused = mode == TOMO_MODE_PHYSICAL && !resident &&
-                               (c->form == TOMO_FORM_FUSED || c->form == TOMO_FORM_RESIDENT);
-            const bool multi = mode == TOMO_MODE_PHYSICAL && !fused && !resident && n_inner > 0 && nside > 0;
+            bool fused = mode == TOMO_MODE_PHYSICAL && !resident &&
+                         (c->form == TOMO_FORM_FUSED || c->form == TOMO_FORM_RESIDENT);
+            bool multi = mode == TOMO_MODE_PHYSICAL && !fused && !resident && n_inner > 0 && nside > 0;
             // per-class cell-steps and bytes of block o (kernel timing only)
             auto block_work = [&](int o, int k, double& cells, double& bytes) {
                 cells = bytes = 0;
@@ -786,12 +787,27 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                     HIPCHK(launch_zero(st.d_flags, (sizeof(int) * st.nres + 15) & ~(size_t)15, c->stream));
                     hipEvent_t a = c->prof ? pool_event(c) : nullptr, b = c->prof ? pool_event(c) : nullptr;
                     if (a && b) HIPCHK(hipEventRecord(a, c->stream));
-                    HIPCHK(launch_tres(T, S, mode, st.d_res, st.nres, d_it, st.d_sims, st.d_flags,
-                                       res_err(st), kResSpinLimit, c->stream));
-                    if (a && b) {
-                        HIPCHK(hipEventRecord(b, c->stream));
-                        c->pending.push_back({a, b, TOMO_KERNEL_RESIDENT, ucells, ubytes, true});
-                        first = a;
+                    const hipError_t le = launch_tres(T, S, mode, st.d_res, st.nres, d_it, st.d_sims, st.d_flags,
+                                                      res_err(st), kResSpinLimit, c->stream);
+                    if (le == hipSuccess) {
+                        if (a && b) {
+                            HIPCHK(hipEventRecord(b, c->stream));
+                            c->pending.push_back({a, b, TOMO_KERNEL_RESIDENT, ucells, ubytes, true});
+                            first = a;
+                        }
+                    } else {
+                        // The device refused the cooperative launch (one process
+                        // at a time may hold a device's cooperative queue), so
+                        // nothing was queued: this epoch and the context's later
+                        // ones run the fused form, whose results are the same.
+                        (void)hipGetLastError();
+                        for (hipEvent_t e : {a, b})
+                            if (e) c->ev_pool.push_back(e);
+                        c->res_refused = true;
+                        resident = false;
+                        fused = mode == TOMO_MODE_PHYSICAL;
+                        multi = false;
+                        ucells = ubytes = 0;
                     }
                 }
                 for (int o = 0; !resident && o < nl; ++o) {
@@ -1651,6 +1667,7 @@ int tomo_get_stats(tomo_ctx* c, tomo_stats* out) {
     int rc = drain_timings(c);
     *out = c->stats;
     out->requests = (int64_t)c->reqs.size();
+    out->resident_refused = c->res_refused ? 1 : 0;
     return rc;
 }
 
