@@ -433,11 +433,22 @@ def run_ref_compat(ctx, sched, n_eval=2, ref_steps=250):
         # vector's excitations advance together)
         ctx.fitness(pid, vecs[0], measurement=target)  # warm-up
         t0 = time.perf_counter()
-        for v in vecs[1:]:
-            ctx.fitness(pid, v, measurement=target)
+        fits = [ctx.fitness(pid, v, measurement=target) for v in vecs[1:]]
         dt = (time.perf_counter() - t0) / n_eval
         out[name] = {"s_per_eval": round(dt, 4), "evals_per_s": round(1 / dt, 3),
                      "Mcell_steps_per_s": round(cell_steps(cfg) / dt / 1e6, 1)}
+        # the same evaluations with HIP events on every launch: per-kernel
+        # launch counts and durations (a cooperative launch is timed by the
+        # events around it on its stream)
+        ctx.profile(True)
+        fits2 = [ctx.fitness(pid, v, measurement=target) for v in vecs[1:]]
+        st = ctx.stats()
+        ctx.profile(False)
+        assert fits2 == fits
+        out[name]["kernels_per_eval"] = {
+            k: {"launches": st[k]["launches"] / n_eval, "avg_us": round(st[k]["ms"] * 1e3 / st[k]["launches"], 2)}
+            for k in st if isinstance(st[k], dict) and k != "update" and st[k]["launches"]}
+        out[name]["resident_refused"] = st["resident_refused"]
     try:
         from oracle.binding import Reference
         if Reference.available():
