@@ -49,7 +49,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")
+UPDATE_KERNELS = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres", "k_twave")
 
 
 def _kernel_roofline(name, k):

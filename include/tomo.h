@@ -152,6 +152,12 @@ const char *tomo_last_error(void);
  * form instead, and so does a context the device refuses a cooperative launch
  * (tomo_stats.resident_refused). */
 #define TOMO_FORM_RESIDENT 3
+/* TOMO_FORM_TILE: one launch per time block in which each workgroup of 8 waves
+ * loads a whole tile (a 64-lane band x 8 * seg_rows rows, seg_rows 4, 5 or 6
+ * rows per wave) into registers, advances it tsteps steps (2, 4, 6 or 8) with
+ * the waves coupled through LDS, and stores its owned cells.  Physical mode;
+ * reference-compatible simulations and unsupported pairs use the fused form. */
+#define TOMO_FORM_TILE 4
 typedef struct tomo_schedule {
     int32_t tsteps, seg_rows, edge_waves, inner_waves, chunk, grid_mult, form, edge_rows;
     /* Output rows of an interior tile of a physical-mode simulation; 0 = seg_rows
@@ -192,7 +198,8 @@ enum {
     TOMO_KERNEL_FITNESS = 7, /* k_fitness: FP64 fitness reduction */
     TOMO_KERNEL_FUSED = 8,   /* k_tfused: every tile class in one launch (TOMO_FORM_FUSED) */
     TOMO_KERNEL_RESIDENT = 9, /* k_tres: a whole epoch in one cooperative launch (TOMO_FORM_RESIDENT) */
-    TOMO_KERNEL_COUNT = 10
+    TOMO_KERNEL_TILE = 10,    /* k_twave: register tiles, one launch per time block (TOMO_FORM_TILE) */
+    TOMO_KERNEL_COUNT = 11
 };
 typedef struct tomo_kernel_stats {
     int64_t launches;

@@ -93,11 +93,14 @@ def test_phys_fields_vs_oracle(ctx, oracle):
                                    (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 8), (2, 5, 2, 32, 4, 4, t.FORM_STREAMS, 2, 12),
                                    (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 14),
                                    (4, 8, 1, 32, 1, 16, t.FORM_RESIDENT), (8, 12, 1, 64, 1, 16, t.FORM_RESIDENT),
-                                   (2, 12, 1, 32, 1, 16, t.FORM_RESIDENT), (4, 12, 1, 64, 1, 16, t.FORM_RESIDENT)])
+                                   (2, 12, 1, 32, 1, 16, t.FORM_RESIDENT), (4, 12, 1, 64, 1, 16, t.FORM_RESIDENT),
+                                   (4, 4, 1, 64, 1, 16, t.FORM_TILE), (2, 5, 1, 32, 1, 16, t.FORM_TILE),
+                                   (6, 4, 1, 60, 1, 16, t.FORM_TILE), (8, 6, 1, 64, 1, 16, t.FORM_TILE),
+                                   (4, 6, 1, 32, 1, 16, t.FORM_TILE)])
 def test_schedule_does_not_change_results(oracle, sched):
     """Fused step count, tile heights (boundary and interior), waves per
-    workgroup, DFT chunk and the launch form (streams, fused, resident) only
-    reorder work: every schedule gives the same fields and accumulators."""
+    workgroup, DFT chunk and the launch form (streams, fused, resident, tile)
+    only reorder work: every schedule gives the same fields and accumulators."""
     cfg = phys_small(steps=200)
     base = t.Context()
     base.set_schedule(4, 32, 4, 32, form=t.FORM_STREAMS)
@@ -266,7 +269,7 @@ def test_autotune_installs_a_schedule_that_keeps_results(oracle):
     c.close()
 
 
-@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("form", [1, 2, 3, 4])
 def test_kernel_stats_account_for_every_cell_step(form):
     """Per-kernel timing: the update kernels' cell-steps add up to the
     simulations' cell-steps, every tile class of a split grid runs (in its
@@ -274,19 +277,19 @@ def test_kernel_stats_account_for_every_cell_step(form):
     cell-step stay within the 25..49 bytes the formula allows."""
     cfg = phys_small(steps=96, n=256, n_exc=2)
     c = t.Context()
-    c.set_schedule(4, 16 if form != 3 else 8, 2, 32, 2, 2, form)
+    c.set_schedule(4, {3: 8, 4: 4}.get(form, 16), 2, 32, 2, 2, form)
     c.target_farfield(cfg)  # the target's own simulations stay out of the counters
     c.profile(True)
     c.eval(cfg, PHYS_PARAMS)
     st = c.stats()
-    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")]
+    upd = [st[k] for k in ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres", "k_twave")]
     want = float(cfg.nx) * cfg.ny * cfg.steps * cfg.n_exc
     assert sum(k["cell_steps"] for k in upd) == want
     assert st["update"]["cell_steps"] == want
-    ran = {1: ("k_tedge", "k_trow", "k_tcol", "k_tinner"), 2: ("k_tfused",), 3: ("k_tres",)}[form]
+    ran = {1: ("k_tedge", "k_trow", "k_tcol", "k_tinner"), 2: ("k_tfused",), 3: ("k_tres",), 4: ("k_twave",)}[form]
     for k in ran + ("k_fold", "k_project_mfma", "k_fitness"):
         assert st[k]["launches"] > 0 and st[k]["ms"] > 0, k
-    for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres")) - set(ran):
+    for k in set(("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_tfused", "k_tres", "k_twave")) - set(ran):
         assert st[k]["launches"] == 0, k
     b = sum(k["alg_bytes"] for k in upd) / want
     assert 25.0 < b < 49.0

@@ -39,6 +39,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 FORM_STREAMS = 1
 FORM_FUSED = 2
 FORM_RESIDENT = 3
+FORM_TILE = 4
 
 SEARCH_DE = 1
 SEARCH_PSO = 2

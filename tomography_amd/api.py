@@ -41,7 +41,7 @@ class Schedule(_c.Structure):
 
 
 KERNEL_NAMES = ("k_tblock", "k_tedge", "k_trow", "k_tcol", "k_tinner", "k_fold", "k_project_mfma", "k_fitness",
-                "k_tfused", "k_tres")
+                "k_tfused", "k_tres", "k_twave")
 
 
 class KernelStats(_c.Structure):

@@ -32,6 +32,15 @@ hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchI
 hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
                          int ordinal, int waves, int grid, hipStream_t st);
 
+// Tile update form: one workgroup of kTileWaveCount waves per TileItem holds a
+// 64-lane band x (kTileWaveCount * rows_per_wave) rows in registers and
+// advances it one time block (T steps); every tile row span y1 - y0 must be
+// at most kTileWaveCount * rows_per_wave - 2T.
+constexpr int kTileWaveCount = 8;
+bool twave_supported(int T, int rows_per_wave);
+hipError_t launch_twave(int T, int rows_per_wave, const TileItem* tiles, int ntiles, const LaunchItem* items,
+                        const DevSim* sims, int ordinal, int grid, hipStream_t st);
+
 // Resident update form: one cooperative launch advances every simulation of
 // `items` by its epoch (all time blocks), one workgroup per ResTile, all
 // co-resident.  `done` (ntiles counters) must start at zero; `err` is set to 1

@@ -1095,6 +1095,164 @@ __global__ __launch_bounds__(kResWaves * kWave, 1) void k_tres(const ResTile* __
         }
     }
 }
+
+// ---------------------------------------------------------------- register-tile update
+// Tile launch form (TOMO_FORM_TILE): one workgroup of kTileWaves waves per
+// tile and time block.  The workgroup loads a 64-lane band (owned columns
+// [x0, x0 + 64 - 2T), T ghost lanes on each side) times the rows
+// [y0 - T, y1 + T) into registers, RW rows per wave, all at once, advances
+// them T steps with the resident kernel's stage arithmetic (so every form
+// gives bit-identical fields), and stores the owned rows.  Per step the waves
+// couple through LDS with two barriers (each publishes its bottom row's Ez^n,
+// then its top row's Hx^{n+1/2}); inside a wave the RW rows of a half-step are
+// independent, so the stage chains of a wave overlap.  Unlike the row sweeps
+// no row waits on the one before it, and unlike the resident form there is no
+// exchange between tiles: ghost rows and lanes (stale after T steps) are
+// reloaded with the whole tile by the next launch.  CPML psi live in
+// registers with the row.
+constexpr int kTileWaves = kTileWaveCount;
+
+template <int T, int RW>
+__global__ __launch_bounds__(kTileWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_twave(const TileItem* __restrict__ tiles, int ntiles,
+                                                             const LaunchItem* __restrict__ items,
+                                                             const DevSim* __restrict__ sims, int ordinal) {
+    __shared__ float4 lc4[TOMO_MAX_MAT * 2];
+    __shared__ float lcp[8 * TOMO_MAX_NC];
+    __shared__ float ezb[kTileWaves][kWave];
+    __shared__ float hxt[kTileWaves][kWave];
+    __shared__ float4 lpsi[kTileWaves][RW][kWave];  // {pex, phx, pey, phy} of general waves
+    constexpr int ID = TOMO_MAX_NC - 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const TileItem tile = tiles[ti];
+        const LaunchItem it = items[tile.item];
+        const int nst = min(T, it.nsteps - ordinal * T);
+        if (nst <= 0) continue;  // uniform over the workgroup
+        const DevSim* Sp = sims + it.sim;
+        __syncthreads();  // the previous tile's tables are no longer read
+        fill_edge<T, false>(Sp, lc4, lcp, nullptr);
+        __syncthreads();
+        SimView S;
+        S.f = (gfloat*)Sp->fbase; S.jp = (gfloat*)Sp->jbase; S.xq = (gfloat*)Sp->xbase; S.yq = (gfloat*)Sp->ybase;
+        S.mat = (gcu8*)Sp->mat; S.span = (kcint*)Sp->span; S.samp = (gfloat*)Sp->samp; S.srcw = (kcfloat*)Sp->srcw;
+        S.ps = Sp->ps; S.js = Sp->js; S.xs = Sp->xs; S.ys = Sp->ys;
+        S.par = it.parity0 ^ (ordinal & 1);
+        S.nx = Sp->nx; S.ny = Sp->ny; S.pitch = Sp->pitch; S.nc = Sp->nc; S.D = Sp->D; S.P = Sp->P;
+        S.L1 = Sp->L1; S.L2 = Sp->L2; S.L3 = Sp->L3; S.cx_hi = Sp->cx_hi; S.cy_lo = Sp->cy_lo;
+        S.sx = Sp->sx; S.sy = Sp->sy; S.flags = Sp->flags; S.K = Sp->chunk;
+        S.chx = Sp->chx; S.chy = Sp->chy; S.chx_d = Sp->chx_d; S.chy_d = Sp->chy_d; S.dxf = Sp->dxf;
+        S.dyf = Sp->dyf;
+        const int nx = S.nx, ny = S.ny, nc = S.nc, D = S.D;
+        const int x0 = tile.x0 - T;  // first lane's column
+        const int x = x0 + lane;
+        const bool xin = (x >= 0 && x <= nx);
+        const bool xown = xin && lane >= T && x < min(tile.x0 + kWave - 2 * T, tile.xe);
+        WaveFlags wf;
+        wf.xs = (x0 <= nc) || (x0 + kWave - 1 >= nx - nc);
+        wf.ccol = (x0 <= D + 1 && x0 + kWave - 1 >= D) || (x0 <= nx - D && x0 + kWave - 1 >= nx - D - 1);
+        wf.src = (S.sx >= x0 && S.sx <= x0 + kWave - 1);
+        ResLane L;
+        {
+            const int ixe = (x >= 1 && x <= nc) ? nc - x : ((x >= nx - nc && x <= nx - 1) ? x - nx + nc : ID);
+            const int ixh = (x >= 0 && x < nc) ? nc - 1 - x : ((x >= nx - nc && x < nx) ? x - nx + nc : ID);
+            L.kxe = CP(2, ixe); L.axe = CP(4, ixe); L.bxe = CP(5, ixe);
+            L.kxh = CP(3, ixh); L.axh = CP(6, ixh); L.bxh = CP(7, ixh);
+            L.hx_ok = (x >= 0 && x < nx);
+            L.ex_ok = (x >= 1 && x <= nx - 1);
+        }
+        const int y0 = tile.y0, y1 = tile.y1;
+        const int yb = y0 - T + wv * RW;  // this wave's first row
+        // Clean waves (every row and lane clear of CPML strips, contour rows and
+        // columns, PEC and the grid's end) take the bare Yee/Debye stage.
+        const int cbot = max(nc, D + 1) + 1, ctop = min(ny - nc, ny - D - 1) - 1;
+        const int cl = max(nc, D + 1) + 1, cr = min(nx - nc, nx - D - 1) - 1;
+        const bool clean = yb >= cbot && yb + RW - 1 <= ctop && x0 >= cl && x0 + kWave - 1 <= cr;
+        RCell rows[RW];
+        float4* psi = lpsi[wv][0] + lane;  // row i at psi[i * kWave]
+        auto put = [&](int i, const Cell& c) {
+            rows[i].ez = c.ez; rows[i].hx = c.hx; rows[i].hy = c.hy; rows[i].jp = c.jp; rows[i].mat = c.mat;
+            if (!clean) psi[i * kWave] = make_float4(c.pex, c.phx, c.pey, c.phy);
+        };
+        auto get = [&](int i) {
+            Cell c;
+            c.ez = rows[i].ez; c.hx = rows[i].hx; c.hy = rows[i].hy; c.jp = rows[i].jp; c.mat = rows[i].mat;
+            const float4 q = clean ? make_float4(0.f, 0.f, 0.f, 0.f) : psi[i * kWave];
+            c.pex = q.x; c.phx = q.y; c.pey = q.z; c.phy = q.w;
+            return c;
+        };
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            Cell c;
+            load_cell(S, lc4, yb + i, x, xin, wf, c);
+            put(i, c);
+        }
+        const int step0 = it.step0 + ordinal * T, k0 = ordinal * T;
+        for (int t = 0; t < nst; ++t) {
+            const int n = step0 + t;
+            ezb[wv][lane] = rows[0].ez;
+            __syncthreads();
+            const float ez_top = wv + 1 < kTileWaves ? ezb[wv + 1][lane] : 0.f;
+            // H half-step of the top row first: the wave above needs its Hx
+            float htx, hty;
+            if (clean) {
+                const RCell& tp = rows[RW - 1];
+                const float dEx = lane_next(tp.ez) - tp.ez;
+                const float dEy = ez_top - tp.ez;
+                hty = tp.hy + S.chy * dEx;
+                htx = tp.hx - S.chx * dEy;
+            } else {
+                Cell tc = get(RW - 1);
+                res_h(S, lcp, L, tc, ez_top, yb + RW - 1, htx, hty);
+                psi[(RW - 1) * kWave] = make_float4(tc.pex, tc.phx, tc.pey, tc.phy);
+            }
+            hxt[wv][lane] = htx;
+            __syncthreads();
+            float hxp = wv > 0 ? hxt[wv - 1][lane] : 0.f;
+            if (clean) {
+                const float chx = S.chx, chy = S.chy;
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    RCell& c = rows[i];
+                    float hx = htx, hy = hty;
+                    if (i + 1 < RW) {
+                        const float dEx = lane_next(c.ez) - c.ez;
+                        const float dEy = rows[i + 1].ez - c.ez;
+                        hy = c.hy + chy * dEx;
+                        hx = c.hx - chx * dEy;
+                    }
+                    const float dHy = hy - lane_prev(hy);
+                    const float dHx = hx - hxp;
+                    hxp = hx;
+                    const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
+                    float en = qa.x * c.ez + qa.y * dHy - qa.y * dHx - qb.x * c.jp;
+                    if (wf.src && yb + i == S.sy && x == S.sx) en += S.srcw[n];
+                    c.jp = qb.y * c.jp + qb.z * (en - c.ez);
+                    c.ez = en;
+                    c.hx = hx;
+                    c.hy = hy;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    const int y = yb + i;
+                    Cell c = get(i);
+                    float hx = htx, hy = hty;
+                    if (i + 1 < RW) res_h(S, lcp, L, c, rows[i + 1].ez, y, hx, hy);
+                    res_e(S, lc4, lcp, L, c, hx, hy, hxp, y, x, n, wf.src);
+                    put(i, c);
+                    if (y >= y0 && y < y1 && xown) write_samples(S, wf, c, x, y, k0 + t);
+                }
+            }
+        }
+        // the owned rows to plane par ^ 1
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const int y = yb + i;
+            if (y >= y0 && y < y1 && xown) store_cell(S, lc4, y, x, wf, get(i));
+        }
+    }
+}
 #undef CP
 
 // Row-edge tiles of physical-mode simulations: the top and bottom rows
@@ -1707,7 +1865,7 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
 // in FP32 and added to the long-running accumulator with Kahan compensation.
 // Block (x, item, z) folds contour points [256 x, 256 x + 256) of one item
 // for frequencies [z fpb, z fpb + fpb), kFoldGroup at a time.  Few items
-// (one simulation) take fpb = kFoldGroup = 2, so the frequencies spread over
+// (one simulation) take fpb = kFoldGroup = 1, so the frequencies spread over
 // more blocks; many items take fpb = all frequencies with kFoldGroup = 8, so
 // each sample is read once per 8 frequencies.  Every frequency's sum runs in
 // the same order either way.
@@ -2168,6 +2326,24 @@ hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchI
     return hipErrorInvalidValue;
 }
 
+bool twave_supported(int T, int rows_per_wave) {
+    return (T == 2 || T == 4 || T == 6 || T == 8) && (rows_per_wave == 4 || rows_per_wave == 5 || rows_per_wave == 6) &&
+           kTileWaves * rows_per_wave > 2 * T;
+}
+
+hipError_t launch_twave(int T, int rows_per_wave, const TileItem* tiles, int ntiles, const LaunchItem* items,
+                        const DevSim* sims, int ordinal, int grid, hipStream_t st) {
+    dim3 g(grid), b(kTileWaves * kWave);
+#define TW(TT, RR)                                                                                   \
+    if (T == TT && rows_per_wave == RR) {                                                            \
+        TIMED_LAUNCH((k_twave<TT, RR>), g, b, 0, st, tiles, ntiles, items, sims, ordinal);           \
+        return hipGetLastError();                                                                    \
+    }
+    TW(2, 4) TW(2, 5) TW(2, 6) TW(4, 4) TW(4, 5) TW(4, 6) TW(6, 4) TW(6, 5) TW(6, 6) TW(8, 4) TW(8, 5) TW(8, 6)
+#undef TW
+    return hipErrorInvalidValue;
+}
+
 bool tres_supported(int T, int rows_per_wave) {
     return (T == 2 || T == 4 || T == 8) && (rows_per_wave == 8 || rows_per_wave == 12);
 }
@@ -2219,13 +2395,13 @@ hipError_t launch_fold(bool fp64, const LaunchItem* items, int nitems, const Dev
     const int bx = (max_p + 255) / 256;
     // fewer than about one block per CU: spread the frequencies too
     const bool narrow = bx * nitems < 256;
-    const int fpb = narrow ? 2 : max_f;
+    const int fpb = narrow ? 1 : max_f;
     dim3 g(bx, nitems, (max_f + fpb - 1) / fpb), b(256);
     if (fp64) {
-        if (narrow) TIMED_LAUNCH((k_fold<true, 2>), g, b, 0, st, items, sims, fpb);
+        if (narrow) TIMED_LAUNCH((k_fold<true, 1>), g, b, 0, st, items, sims, fpb);
         else TIMED_LAUNCH((k_fold<true, 8>), g, b, 0, st, items, sims, fpb);
     } else {
-        if (narrow) TIMED_LAUNCH((k_fold<false, 2>), g, b, 0, st, items, sims, fpb);
+        if (narrow) TIMED_LAUNCH((k_fold<false, 1>), g, b, 0, st, items, sims, fpb);
         else TIMED_LAUNCH((k_fold<false, 8>), g, b, 0, st, items, sims, fpb);
     }
     return hipGetLastError();

@@ -478,27 +478,31 @@ int n_bands(int nx, int T) { return (nx + 1 + (64 - 2 * T) - 1) / (64 - 2 * T); 
 // full-width bands of the clean interior go to the interior kernel (physical
 // mode only).  Interior bands start exactly where the clean zone starts, so
 // the edge frame is one narrow band wide on each side.
+// Algorithmic bytes of the cells a tile owns inside [0, nx) x [0, ny), added
+// to its class: Ez, Hx, Hy read and written once (24) and the material id (1)
+// on every cell, Jp (8) on Debye cells, the psi pair (16) on CPML-strip cells.
+void account_tile(Slot* sl, int k, const TileItem& t) {
+    const DevSim& d = sl->d;
+    const int nx = d.nx, ny = d.ny, nc = d.nc;
+    const int x0 = std::max(t.x0, 0), x1 = std::min(t.xe, nx);
+    const int y0 = std::max(t.y0, 0), y1 = std::min(t.y1, ny);
+    if (x1 <= x0 || y1 <= y0) return;
+    const double cells = (double)(x1 - x0) * (y1 - y0);
+    double jp = 0;
+    for (int y = y0; y < y1; ++y)
+        jp += std::max(0, std::min(x1, sl->su.span[3 * y + 1]) - std::max(x0, sl->su.span[3 * y]));
+    const double cx = std::max(0, std::min(x1, nx - nc) - std::max(x0, nc));
+    const double cy = std::max(0, std::min(y1, ny - nc) - std::max(y0, nc));
+    sl->cls_cells[k] += cells;
+    sl->cls_bytes[k] += 25.0 * cells + 8.0 * jp + 16.0 * (cells - cx * cy);
+}
+
 void build_tiles(Slot* sl, int item, int T, int S, int Si, int W, int Wi, int ER, std::vector<TileItem>& edge,
                  std::vector<TileItem>& rowedge, std::vector<TileItem>& coledge, std::vector<TileItem>& inner) {
     const DevSim& d = sl->d;
     for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
     const int nx = d.nx, ny = d.ny, nc = d.nc, D = d.D, V = 64 - 2 * T;
-    // Algorithmic bytes of the cells a tile owns inside [0, nx) x [0, ny):
-    // Ez, Hx, Hy read and written once (24) and the material id (1) on every
-    // cell, Jp (8) on Debye cells, the psi pair (16) on CPML-strip cells.
-    auto account = [&](int k, const TileItem& t) {
-        const int x0 = std::max(t.x0, 0), x1 = std::min(t.xe, nx);
-        const int y0 = std::max(t.y0, 0), y1 = std::min(t.y1, ny);
-        if (x1 <= x0 || y1 <= y0) return;
-        const double cells = (double)(x1 - x0) * (y1 - y0);
-        double jp = 0;
-        for (int y = y0; y < y1; ++y)
-            jp += std::max(0, std::min(x1, sl->su.span[3 * y + 1]) - std::max(x0, sl->su.span[3 * y]));
-        const double cx = std::max(0, std::min(x1, nx - nc) - std::max(x0, nc));
-        const double cy = std::max(0, std::min(y1, ny - nc) - std::max(y0, nc));
-        sl->cls_cells[k] += cells;
-        sl->cls_bytes[k] += 25.0 * cells + 8.0 * jp + 16.0 * (cells - cx * cy);
-    };
+    auto account = [&](int k, const TileItem& t) { account_tile(sl, k, t); };
     // column ranges [a, b) split into bands of V and grouped w bands per tile
     auto add = [&](int k, std::vector<TileItem>& v, int a, int b, int y0, int y1, int w) {
         for (int x = a; x < b; x += w * V) {
@@ -574,6 +578,29 @@ void build_res_tiles(const Slot* sl, int item, int T, int RW, std::vector<ResTil
                     const int X = bx + dx, Y = ry + dy;
                     t.nb[k++] = (X >= 0 && X < nbx && Y >= 0 && Y < nby) ? base + Y * nbx + X : -1;
                 }
+            out.push_back(t);
+        }
+}
+
+// Tiles of the tile form: one band of V = 64 - 2T owned columns times up to
+// kTileWaveCount * RW - 2T owned rows (rows split evenly), one workgroup each.
+void build_wave_tiles(Slot* sl, int item, int T, int RW, std::vector<TileItem>& out) {
+    const DevSim& d = sl->d;
+    for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
+    const int V = 64 - 2 * T, R = kTileWaveCount * RW - 2 * T;
+    const int nbx = (d.nx + 1 + V - 1) / V, nby = (d.ny + 1 + R - 1) / R;
+    const int rb = (d.ny + 1 + nby - 1) / nby;
+    for (int ry = 0; ry < nby; ++ry)
+        for (int bx = 0; bx < nbx; ++bx) {
+            TileItem t{};
+            t.item = item;
+            t.x0 = bx * V;
+            t.nband = 1;
+            t.y0 = ry * rb;
+            t.y1 = std::min(t.y0 + rb, d.ny + 1);
+            t.xe = std::min(t.x0 + V, d.nx + 1);
+            t.cls = TILE_INNER;
+            account_tile(sl, TILE_INNER, t);
             out.push_back(t);
         }
 }
@@ -676,10 +703,14 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
             // resident form: every tile of the set in one cooperative launch,
             // when they fit on the device at once (else the fused form runs)
             const bool want_res = c->form == TOMO_FORM_RESIDENT && tres_supported(T, S) && !st.fixed;
+            // tile form: whole register tiles, one workgroup each, in place of the four classes
+            const bool wave = c->form == TOMO_FORM_TILE && mode == TOMO_MODE_PHYSICAL && twave_supported(T, S);
             std::vector<ResTile> trs;
             if (rebuild) {
-                for (size_t i = 0; i < ms.size(); ++i)
-                    build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
+                for (size_t i = 0; i < ms.size(); ++i) {
+                    if (wave) build_wave_tiles(ms[i], (int)i, T, S, tn);
+                    else build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
+                }
                 if (want_res) {
                     for (size_t i = 0; i < ms.size(); ++i) build_res_tiles(ms[i], (int)i, T, S, trs);
                     if ((int)trs.size() > tres_max_tiles(T, S, mode, c->cus)) trs.clear();
@@ -754,9 +785,9 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
             const int cnt[3] = {n0, nr, ncl};
             int nside = 0;
             for (int k = 0; k < 3; ++k) nside += cnt[k] > 0;
-            bool fused = mode == TOMO_MODE_PHYSICAL && !resident &&
-                         (c->form == TOMO_FORM_FUSED || c->form == TOMO_FORM_RESIDENT);
-            bool multi = mode == TOMO_MODE_PHYSICAL && !fused && !resident && n_inner > 0 && nside > 0;
+            bool fused = mode == TOMO_MODE_PHYSICAL && !resident && !wave &&
+                         (c->form == TOMO_FORM_FUSED || c->form == TOMO_FORM_RESIDENT || c->form == TOMO_FORM_TILE);
+            bool multi = mode == TOMO_MODE_PHYSICAL && !fused && !resident && !wave && n_inner > 0 && nside > 0;
             // per-class cell-steps and bytes of block o (kernel timing only)
             auto block_work = [&](int o, int k, double& cells, double& bytes) {
                 cells = bytes = 0;
@@ -822,6 +853,16 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                         HIPCHK(launch_tblock(T, mode, st.d_tiles, n_edge, d_it, st.d_sims, o, W,
                                              std::min(n_edge, cap), c->stream));
                         lt.done(TOMO_KERNEL_GENERAL, cells[0], bytes[0]);
+                        if (!first) first = lt.a;
+                        continue;
+                    }
+                    if (wave) {
+                        const int nt = st.ntiles;
+                        LaunchTimer lt(c);
+                        HIPCHK(launch_twave(T, S, st.d_tiles, nt, d_it, st.d_sims, o, std::min(nt, 2 * cap),
+                                            c->stream));
+                        lt.done(TOMO_KERNEL_TILE, cells[0] + cells[1] + cells[2] + cells[3],
+                                bytes[0] + bytes[1] + bytes[2] + bytes[3]);
                         if (!first) first = lt.a;
                         continue;
                     }
@@ -1524,7 +1565,8 @@ static int set_schedule_locked(tomo_ctx* c, const tomo_schedule& s) {
     const int ER = s.edge_rows ? s.edge_rows : c->ER;
     if (!tblock_supported(T)) return fail(TOMO_E_INVALID, "unsupported fused step count");
     if (W < 1 || W > 4 || Wi < 1 || Wi > 4 || S < 4 || S > 4096 || K < T || K > 64 || K % T || G < 1 || G > 64 ||
-        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED && F != TOMO_FORM_RESIDENT) || ER < 1 || ER > 4096 ||
+        (F != TOMO_FORM_STREAMS && F != TOMO_FORM_FUSED && F != TOMO_FORM_RESIDENT && F != TOMO_FORM_TILE) ||
+        ER < 1 || ER > 4096 ||
         (s.inner_rows != 0 && (s.inner_rows < 4 || s.inner_rows > 4096)))
         return fail(TOMO_E_INVALID, "bad schedule");
     c->T = T; c->S = S; c->W = W; c->Wi = Wi; c->K = K; c->grid_mult = G; c->form = F; c->ER = ER;
@@ -1614,15 +1656,16 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     if (!rc) rc = trial(cur);
     // Seeds measured on MI355X, so that the descent does not start from a
     // poor corner of the space: one simulation (fused, short tiles) and many
-    // (four streams, tall tiles).
+    // (four streams, tall tiles; T = 6 where the chunk allows it).
     if (tc.mode == TOMO_MODE_PHYSICAL) {
-        const tomo_schedule seeds[2] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 14},
-                                        {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 8, 0}};
+        const tomo_schedule seeds[3] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 14},
+                                        {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0},
+                                        {6, 48, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0}};
         for (const tomo_schedule& sd : seeds)
             if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
     }
-    const int Ts[] = {1, 2, 4, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
-    const int ERs[] = {1, 2, 3, 4, 8, 16}, SIs[] = {0, 8, 12, 14, 16, 24};
+    const int Ts[] = {1, 2, 4, 6, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 48, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
+    const int ERs[] = {1, 2, 3, 4, 8, 16, 24}, SIs[] = {0, 8, 12, 14, 16, 24};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
