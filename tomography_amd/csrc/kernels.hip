@@ -1925,8 +1925,9 @@ __global__ __launch_bounds__(256) void k_fold(const LaunchItem* __restrict__ ite
 #pragma unroll
             for (int q = 0; q < kFoldGroup; ++q) jr[q] = ji[q] = mr[q] = mi[q] = 0.f;
             // unrolled so that several steps' samples are in flight at once
-            // (the accumulation order is unchanged)
-#pragma unroll 8
+            // (the accumulation order is unchanged); one frequency per thread
+            // leaves the registers for 20 steps' loads
+#pragma unroll (kFoldGroup == 1 ? 20 : 8)
             for (int k = 0; k < nk; ++k) {
                 const float h = hs[(size_t)k * P];
                 const float e = (e0[(size_t)k * P] + e1[(size_t)k * P]) / 2;
