@@ -582,6 +582,27 @@ void build_res_tiles(const Slot* sl, int item, int T, int RW, std::vector<ResTil
         }
 }
 
+// Workgroups of a launch are dealt to the eight XCDs round-robin, and a
+// kernel's grid is a multiple of 8, so tile p runs on XCD p mod 8.  Order a
+// class's tile list (simulation-major, each simulation's tiles row-major) so
+// that XCD g walks the g-th contiguous eighth of it: vertically and
+// horizontally adjacent tiles, which read each other's halo rows and
+// columns, then share one L2.  The last eighth is padded with empty tiles
+// (nband 0), which every kernel skips.
+void xcd_order(std::vector<TileItem>& v) {
+    if (v.size() < 16) return;
+    const size_t L = (v.size() + 7) / 8;
+    std::vector<TileItem> out(8 * L);
+    for (size_t k = 0; k < L; ++k)
+        for (size_t g = 0; g < 8; ++g) {
+            const size_t src = g * L + k;
+            TileItem e = v[src < v.size() ? src : 0];
+            if (src >= v.size()) e.nband = 0;
+            out[8 * k + g] = e;
+        }
+    v.swap(out);
+}
+
 // Tiles of the tile form: one band of V = 64 - 2T owned columns times up to
 // kTileWaveCount * RW - 2T owned rows (rows split evenly), one workgroup each.
 void build_wave_tiles(Slot* sl, int item, int T, int RW, std::vector<TileItem>& out) {
@@ -710,6 +731,12 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                 for (size_t i = 0; i < ms.size(); ++i) {
                     if (wave) build_wave_tiles(ms[i], (int)i, T, S, tn);
                     else build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
+                }
+                if (c->form == TOMO_FORM_STREAMS && ms.size() >= 8) {
+                    xcd_order(te);
+                    xcd_order(tr);
+                    xcd_order(tc);
+                    xcd_order(tn);
                 }
                 if (want_res) {
                     for (size_t i = 0; i < ms.size(); ++i) build_res_tiles(ms[i], (int)i, T, S, trs);
