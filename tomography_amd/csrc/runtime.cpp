@@ -588,7 +588,8 @@ void build_res_tiles(const Slot* sl, int item, int T, int RW, std::vector<ResTil
 // that XCD g walks the g-th contiguous eighth of it: vertically and
 // horizontally adjacent tiles, which read each other's halo rows and
 // columns, then share one L2.  The last eighth is padded with empty tiles
-// (nband 0), which every kernel skips.
+// (nband 0), which every kernel skips.  The fused launch concatenates the
+// classes after ordering each, so every XCD holds an eighth of each class.
 void xcd_order(std::vector<TileItem>& v) {
     if (v.size() < 16) return;
     const size_t L = (v.size() + 7) / 8;
@@ -732,7 +733,7 @@ int run_epochs(tomo_ctx* c, const std::vector<Slot*>& sl, Staging& st) {
                     if (wave) build_wave_tiles(ms[i], (int)i, T, S, tn);
                     else build_tiles(ms[i], (int)i, T, S, c->Si ? c->Si : S, W, c->Wi, c->ER, te, tr, tc, tn);
                 }
-                if (c->form == TOMO_FORM_STREAMS && ms.size() >= 8) {
+                if (c->form == TOMO_FORM_STREAMS || c->form == TOMO_FORM_FUSED) {
                     xcd_order(te);
                     xcd_order(tr);
                     xcd_order(tc);
