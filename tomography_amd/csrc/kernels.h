@@ -28,6 +28,10 @@ hipError_t launch_tcol(int T, const TileItem* tiles, int ntiles, const LaunchIte
                        int ordinal, int waves, int grid, hipStream_t st);
 hipError_t launch_tinner(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
                          int ordinal, int waves, int grid, hipStream_t st);
+// Rows a row-sweep tile streams (its output rows y1 - y0 plus T ghost rows on
+// each side) are at most kMaxTileRows: the sweeps of the interior, row-edge,
+// column-edge and corner tiles hold the rows' dispersive spans one per lane.
+constexpr int kMaxTileRows = 64;
 // All tile classes of physical-mode simulations in one launch (TileItem::cls).
 hipError_t launch_tfused(int T, const TileItem* tiles, int ntiles, const LaunchItem* items, const DevSim* sims,
                          int ordinal, int waves, int grid, hipStream_t st);

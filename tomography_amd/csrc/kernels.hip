@@ -39,6 +39,10 @@ constexpr int kGroupInner = 4, kGroupRow = 1, kGroupCol = 1, kGroupEdge = 1, kGr
 // two interior rows per group measured 0.8% faster than four on 1024^2 with
 // 14-row interior tiles).
 constexpr int kFusedGroupInner = 2, kFusedGroupRow = 1, kFusedGroupCol = 1, kFusedGroupEdge = 1;
+// Row spans: the per-class kernels read them from a per-lane window (SpanWin,
+// 3% faster on a 1024 x 512^2 batch); the fused kernel keeps scalar loads,
+// since the window's registers made its bodies spill more (1024^2: 21.6
+// against 21.1 us per launch).
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding).
 constexpr int kWaitVm0 = 0x0F70;
 
@@ -156,6 +160,47 @@ __device__ __forceinline__ bool row_ystrip_h(const SimView& S, int r) {
     return (r >= 0 && r < S.nc) || (r >= S.ny - S.nc && r < S.ny);
 }
 
+// Dispersive-region span (first column, end column, offset in the compact
+// jp array) of one row.
+struct Span {
+    int lo, hi, off;
+};
+
+// The spans of the rows a wave streams, one row per lane: lane i holds row
+// rb + i.  The window is one vector load per tile; the row loop then reads a
+// row's span with v_readlane, where a scalar load per row would make every row
+// wait for memory (a scalar load's wait also waits for every LDS read in
+// flight).  A tile streams at most 64 rows (its output rows plus T ghost rows
+// on each side: the tile builders keep y1 - y0 <= 64 - 2T).
+struct SpanWin {
+    int lo, hi, off;
+    int rb;
+};
+
+__device__ __forceinline__ SpanWin span_window(const int* __restrict__ span, int rb, int ny) {
+    SpanWin w;
+    const int r = rb + (int)(threadIdx.x & (kWave - 1));
+    w.lo = w.hi = w.off = 0;
+    if (r >= 0 && r <= ny) {
+        const gcint* p = (const gcint*)span + 3 * r;
+        w.lo = p[0];
+        w.hi = p[1];
+        w.off = p[2];
+    }
+    w.rb = rb;
+    return w;
+}
+
+// Span of row r (wave-uniform, 0 <= r <= ny, 0 <= r - rb < 64).
+__device__ __forceinline__ Span span_row(const SpanWin& w, int r) {
+    const int i = r - w.rb;
+    Span s;
+    s.lo = __builtin_amdgcn_readlane(w.lo, i);
+    s.hi = __builtin_amdgcn_readlane(w.hi, i);
+    s.off = __builtin_amdgcn_readlane(w.off, i);
+    return s;
+}
+
 // Element access of load_cell / store_cell.  COH: relaxed atomics at agent
 // scope (coherent across the XCDs' L2 caches without a cache flush), for data
 // that another workgroup of the same launch wrote or will read.
@@ -172,7 +217,7 @@ __device__ __forceinline__ void st_(gfloat* p, float v) {
 
 template <bool COH = false>
 __device__ __forceinline__ void load_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x, bool xin,
-                                          const WaveFlags& wf, Cell& c) {
+                                          const WaveFlags& wf, Cell& c, const SpanWin* sw = nullptr) {
     c = zero_cell();
     // The material id is read unconditionally from the clamped node: a byte
     // loaded under a branch is zero-extended right after the branch, which
@@ -188,9 +233,14 @@ __device__ __forceinline__ void load_cell(const SimView& S, const float4* __rest
         c.hx = ld_<COH>(S.f + in_f + 2 * S.ps + base);
         c.hy = ld_<COH>(S.f + in_f + 4 * S.ps + base);
     }
-    const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (hi > lo && x >= lo && x < hi)
-        c.jp = ld_<COH>(S.jp + S.par * S.js + S.span[3 * r + 2] + x - lo);
+    if (sw) {
+        const Span sp = span_row(*sw, r);
+        if (x >= sp.lo && x < sp.hi) c.jp = ld_<COH>(S.jp + S.par * S.js + sp.off + x - sp.lo);
+    } else {
+        const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
+        if (hi > lo && x >= lo && x < hi)
+            c.jp = ld_<COH>(S.jp + S.par * S.js + S.span[3 * r + 2] + x - lo);
+    }
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
         gfloat* pe = S.xq + S.par * S.xs;
@@ -212,7 +262,7 @@ __device__ __forceinline__ void load_cell(const SimView& S, const float4* __rest
 
 template <bool COH = false>
 __device__ __forceinline__ void store_cell(const SimView& S, const float4* __restrict__ lc4, int r, int x,
-                                           const WaveFlags& wf, const Cell& c) {
+                                           const WaveFlags& wf, const Cell& c, const SpanWin* sw = nullptr) {
     const int base = r * S.pitch + x;
     const int out_f = (S.par ^ 1) * S.ps;
     st_<COH>(S.f + out_f + base, c.ez);
@@ -220,9 +270,14 @@ __device__ __forceinline__ void store_cell(const SimView& S, const float4* __res
         st_<COH>(S.f + out_f + 2 * S.ps + base, c.hx);
         st_<COH>(S.f + out_f + 4 * S.ps + base, c.hy);
     }
-    const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
-    if (hi > lo && x >= lo && x < hi)
-        st_<COH>(S.jp + (S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo, c.jp);
+    if (sw) {
+        const Span sp = span_row(*sw, r);
+        if (x >= sp.lo && x < sp.hi) st_<COH>(S.jp + (S.par ^ 1) * S.js + sp.off + x - sp.lo, c.jp);
+    } else {
+        const int lo = S.span[3 * r], hi = S.span[3 * r + 1];
+        if (hi > lo && x >= lo && x < hi)
+            st_<COH>(S.jp + (S.par ^ 1) * S.js + S.span[3 * r + 2] + x - lo, c.jp);
+    }
     const int nc = S.nc, nx = S.nx, ny = S.ny, w = 2 * nc;
     if (wf.xs) {
         gfloat* pe = S.xq + (S.par ^ 1) * S.xs;
@@ -610,7 +665,7 @@ __device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4*
     }
 }
 
-template <int T, int G, bool BF>
+template <int T, int G, bool BF, bool SW>
 __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcp,
@@ -655,8 +710,9 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         cur[t] = zero_cell();
         hxp[t] = 0.f;
     }
-    auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c); };
-    auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c); };
+    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c, SW ? &sw : nullptr); };
+    auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c, SW ? &sw : nullptr); };
     // FAST: every stage active (nst == T) and no source cell in reach of the tile,
     // so the stage loop carries no per-stage branch.
     auto stepg = [&](auto fast, int r, Cell in) {
@@ -773,7 +829,7 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         fill_edge<T, BF>(Sp, lc4, lcp, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_edge<T, kGroupEdge, BF>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
+        body_edge<T, kGroupEdge, BF, true>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
     }
 }
 
@@ -1287,7 +1343,7 @@ __device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T, int G, bool BF>
+template <int T, int G, bool BF, bool SW>
 __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcy,
@@ -1307,7 +1363,6 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
     gfloat* ph_in = (gfloat*)Sp->ybase + (2 + par) * ys;
     gfloat* ph_out = (gfloat*)Sp->ybase + (2 + (par ^ 1)) * ys;
     gcu8* mat = (gcu8*)Sp->mat;
-    kcint* span = (kcint*)Sp->span;
     const float chx = Sp->chx, chy = Sp->chy, dyf = Sp->dyf;
     const int bx = tile.x0 + wv * V;
     const int x0 = bx - T;
@@ -1323,6 +1378,8 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
     const int y0 = tile.y0, y1 = tile.y1;
     const int step0 = it.step0 + ordinal * T;
     const int k0 = ordinal * T;
+    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, YCell& c) {
         c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
         c.mat = mat[min(max(r, 0), ny) * pitch + x];  // unconditional (see load_cell)
@@ -1333,8 +1390,13 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
             c.hx = fin[2 * ps + b];
             c.hy = fin[4 * ps + b];
         }
-        const int lo = span[3 * r], hi = span[3 * r + 1];
-        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        if constexpr (SW) {
+            const Span sp = span_row(sw, r);
+            if (x >= sp.lo && x < sp.hi) c.jp = jin[sp.off + x - sp.lo];
+        } else {
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        }
         if (r >= 1 && r <= nc) c.pey = pe_in[(r - 1) * pitch + x];
         else if (r >= ny - nc && r <= ny - 1) c.pey = pe_in[(nc + r - (ny - nc)) * pitch + x];
         if (r < nc) c.phy = ph_in[r * pitch + x];
@@ -1355,8 +1417,13 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
                 fout[2 * ps + b] = c.hx;
                 fout[4 * ps + b] = c.hy;
             }
-            const int lo = span[3 * yo], hi = span[3 * yo + 1];
-            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            if constexpr (SW) {
+                const Span sp = span_row(sw, yo);
+                if (x >= sp.lo && x < sp.hi) jout[sp.off + x - sp.lo] = c.jp;
+            } else {
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            }
             if (yo >= 1 && yo <= nc) pe_out[(yo - 1) * pitch + x] = c.pey;
             else if (yo >= ny - nc && yo <= ny - 1) pe_out[(nc + yo - (ny - nc)) * pitch + x] = c.pey;
             if (yo < nc) ph_out[yo * pitch + x] = c.phy;
@@ -1480,7 +1547,7 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         fill_row<T, BF>(Sp, lc4, lcy, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_row<T, kGroupRow, BF>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
+        body_row<T, kGroupRow, BF, true>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
     }
 }
 
@@ -1512,7 +1579,7 @@ __device__ __forceinline__ void fill_col(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool SW>
 __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcx) {
@@ -1531,7 +1598,6 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
     gfloat* ph_in = (gfloat*)Sp->xbase + (2 + par) * xs;
     gfloat* ph_out = (gfloat*)Sp->xbase + (2 + (par ^ 1)) * xs;
     gcu8* mat = (gcu8*)Sp->mat;
-    kcint* span = (kcint*)Sp->span;
     const float chx = Sp->chx, chy = Sp->chy, dxf = Sp->dxf;
     const int bx = tile.x0 + wv * V;
     const int x0 = bx - T;
@@ -1572,6 +1638,8 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
     const bool s_lane = ccol && own;
     const int ylo = max(max(y0, cy_lo), 0), yhi = min(y1, ny - D);  // rows of the contour's x legs
     const int w2 = 2 * nc;
+    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, XCell& c) {
         c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
         c.mat = mat[r * pitch + min(max(x, 0), nx)];  // unconditional (see load_cell)
@@ -1582,8 +1650,13 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
             c.hx = fin[2 * ps + b];
             c.hy = fin[4 * ps + b];
         }
-        const int lo = span[3 * r], hi = span[3 * r + 1];
-        if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        if constexpr (SW) {
+            const Span sp = span_row(sw, r);
+            if (x >= sp.lo && x < sp.hi) c.jp = jin[sp.off + x - sp.lo];
+        } else {
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi) c.jp = jin[span[3 * r + 2] + x - lo];
+        }
         if (je >= 0) c.pex = pe_in[r * w2 + je];
         if (jh >= 0) c.phx = ph_in[r * w2 + jh];
     };
@@ -1602,8 +1675,13 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
                 fout[2 * ps + b] = c.hx;
                 fout[4 * ps + b] = c.hy;
             }
-            const int lo = span[3 * yo], hi = span[3 * yo + 1];
-            if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            if constexpr (SW) {
+                const Span sp = span_row(sw, yo);
+                if (x >= sp.lo && x < sp.hi) jout[sp.off + x - sp.lo] = c.jp;
+            } else {
+                const int lo = span[3 * yo], hi = span[3 * yo + 1];
+                if (x >= lo && x < hi) jout[span[3 * yo + 2] + x - lo] = c.jp;
+            }
             if (je >= 0) pe_out[yo * w2 + je] = c.pex;
             if (jh >= 0) ph_out[yo * w2 + jh] = c.phx;
     };
@@ -1679,7 +1757,7 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         fill_col<T>(Sp, lc4, lcx);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_col<T, kGroupCol>(tile, it, nst, Sp, ordinal, lc4, lcx);
+        body_col<T, kGroupCol, true>(tile, it, nst, Sp, ordinal, lc4, lcx);
     }
 }
 
@@ -1702,7 +1780,7 @@ __device__ __forceinline__ void fill_inner(const DevSim* __restrict__ Sp, float4
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool SW>
 __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4) {
@@ -1716,7 +1794,6 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     gfloat* jin = (gfloat*)Sp->jbase + par * js;
     gfloat* jout = (gfloat*)Sp->jbase + (par ^ 1) * js;
     gcu8* mat = (gcu8*)Sp->mat;
-    kcint* span = (kcint*)Sp->span;
     const float chx = Sp->chx, chy = Sp->chy;
     const int x0 = tile.x0 + wv * V - T;
     const int x = x0 + lane;
@@ -1732,6 +1809,8 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     const unsigned bx = (unsigned)x * 4u;  // x >= 0 on interior tiles
     typedef GAS char gbyte;
     auto at = [&](gfloat* base, unsigned off) -> gfloat& { return *(gfloat*)((gbyte*)base + off); };
+    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, Sp->ny) : SpanWin{};
+    kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, ICell& c) {
         gfloat* row = fin + (size_t)r * pitch;
         c.ez = at(row, bx);
@@ -1739,16 +1818,26 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
         c.hy = at(row + 4 * (size_t)ps, bx);
         c.mat = (mat + (size_t)r * pitch)[(unsigned)x];
         c.jp = 0.f;
-        const int lo = span[3 * r], hi = span[3 * r + 1];
-        if (x >= lo && x < hi) c.jp = at(jin + span[3 * r + 2] - lo, bx);
+        if constexpr (SW) {
+            const Span sp = span_row(sw, r);
+            if (x >= sp.lo && x < sp.hi) c.jp = at(jin + sp.off - sp.lo, bx);
+        } else {
+            const int lo = span[3 * r], hi = span[3 * r + 1];
+            if (x >= lo && x < hi) c.jp = at(jin + span[3 * r + 2] - lo, bx);
+        }
     };
     auto store = [&](int yo, const ICell& c) {
         gfloat* row = fout + (size_t)yo * pitch;
         at(row, bx) = c.ez;
         at(row + 2 * (size_t)ps, bx) = c.hx;
         at(row + 4 * (size_t)ps, bx) = c.hy;
-        const int lo = span[3 * yo], hi = span[3 * yo + 1];
-        if (x >= lo && x < hi) at(jout + span[3 * yo + 2] - lo, bx) = c.jp;
+        if constexpr (SW) {
+            const Span sp = span_row(sw, yo);
+            if (x >= sp.lo && x < sp.hi) at(jout + sp.off - sp.lo, bx) = c.jp;
+        } else {
+            const int lo = span[3 * yo], hi = span[3 * yo + 1];
+            if (x >= lo && x < hi) at(jout + span[3 * yo + 2] - lo, bx) = c.jp;
+        }
     };
     ICell cur[T];
     float hxp[T];
@@ -1812,7 +1901,7 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
         fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_inner<T, kGroupInner>(tile, it, nst, Sp, ordinal, lc4);
+        body_inner<T, kGroupInner, true>(tile, it, nst, Sp, ordinal, lc4);
     }
 }
 
@@ -1851,10 +1940,10 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
         // Boundary tiles set the launch's length (their stages carry the CPML
         // and contour work): their waves issue first on a shared SIMD.
         if (cls != TILE_INNER) __builtin_amdgcn_s_setprio(3);
-        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
-        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
-        else if (cls == TILE_COL) body_col<T, kFusedGroupCol>(tile, it, nst, Sp, ordinal, lc4, ltab);
-        else body_inner<T, kFusedGroupInner>(tile, it, nst, Sp, ordinal, lc4);
+        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF, false>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF, false>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_COL) body_col<T, kFusedGroupCol, false>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else body_inner<T, kFusedGroupInner, false>(tile, it, nst, Sp, ordinal, lc4);
         __builtin_amdgcn_s_setprio(0);
     }
 }

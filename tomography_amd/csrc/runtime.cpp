@@ -502,6 +502,12 @@ void build_tiles(Slot* sl, int item, int T, int S, int Si, int W, int Wi, int ER
     const DevSim& d = sl->d;
     for (int k = 0; k < 4; ++k) sl->cls_cells[k] = sl->cls_bytes[k] = 0;
     const int nx = d.nx, ny = d.ny, nc = d.nc, D = d.D, V = 64 - 2 * T;
+    // A tile streams its output rows plus T ghost rows on each side through
+    // the row sweep, at most 64 (the sweeps hold the rows' dispersive spans
+    // one row per lane): taller segments are split.
+    S = std::min(S, kMaxTileRows - 2 * T);
+    Si = std::min(Si, kMaxTileRows - 2 * T);
+    ER = std::min(ER, kMaxTileRows - 2 * T);
     auto account = [&](int k, const TileItem& t) { account_tile(sl, k, t); };
     // column ranges [a, b) split into bands of V and grouped w bands per tile
     auto add = [&](int k, std::vector<TileItem>& v, int a, int b, int y0, int y1, int w) {
