@@ -43,6 +43,10 @@ constexpr int kFusedGroupInner = 2, kFusedGroupRow = 1, kFusedGroupCol = 1, kFus
 // 3% faster on a 1024 x 512^2 batch); the fused kernel keeps scalar loads,
 // since the window's registers made its bodies spill more (1024^2: 21.6
 // against 21.1 us per launch).
+// Window flags of the row-sweep bodies: kWinSpan reads the rows' dispersive
+// spans from a SpanWin, kWinY the y-strip CPML coefficients from a YWin.
+constexpr int kWinSpan = 1, kWinY = 2;
+constexpr int kStageWin = kWinSpan | kWinY, kFusedWin = kWinY;
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding).
 constexpr int kWaitVm0 = 0x0F70;
 
@@ -91,6 +95,43 @@ __device__ __forceinline__ void row_stream(int y0, int y1, bool own, Load& load,
         const int yo = gl + i - T;
         if (gl + i < re && yo >= y0 && yo < y1 && own) store(yo, R[i]);
     }
+}
+
+// The same rows, with each row's loads issued P rows before the row is
+// computed and no explicit wait: the compiler waits for a row's loads before
+// its stage chain uses them.  While only loads are in flight those waits count
+// down in issue order and keep P rows in flight; once a store is in flight
+// too, a wait drains the counter.  A tile whose output rows (the last
+// y1 - y0 of the streamed rows) number at most P issues every load before its
+// first store, so its rows' memory latencies overlap instead of adding up:
+// the form for short boundary tiles, whose rows each wait a whole memory
+// latency in the grouped sweep.
+template <int T, int P, class C, class Load, class Store, class Step>
+__device__ __forceinline__ void row_ring(int y0, int y1, bool own, Load& load, Store& store, Step& step) {
+    const int rb = y0 - T, re = y1 + T;
+    C ring[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        if (rb + i < re) load(rb + i, ring[i]);
+    for (int r0 = rb; r0 < re; r0 += P) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int r = r0 + i;
+            if (r >= re) break;
+            const C X = ring[i];
+            if (r + P < re) load(r + P, ring[i]);
+            const C R = step(r, X);
+            const int yo = r - T;
+            if (yo >= y0 && yo < y1 && own) store(yo, R);
+        }
+    }
+}
+
+// Row sweep of a tile: grouped (G > 0 rows per wait) or a load ring of -G rows.
+template <int T, int G, class C, class Load, class Store, class Step>
+__device__ __forceinline__ void row_sweep(int y0, int y1, bool own, Load& load, Store& store, Step& step) {
+    if constexpr (G > 0) row_stream<T, G, C>(y0, y1, own, load, store, step);
+    else row_ring<T, -G, C>(y0, y1, own, load, store, step);
 }
 
 // Global-address-space views: loads and stores through these compile to
@@ -199,6 +240,50 @@ __device__ __forceinline__ Span span_row(const SpanWin& w, int r) {
     s.hi = __builtin_amdgcn_readlane(w.hi, i);
     s.off = __builtin_amdgcn_readlane(w.off, i);
     return s;
+}
+
+// The y-strip CPML coefficients of the rows a wave streams, one row per lane
+// (lane i: row rb + i): ike, ae, be of the E row index and ikm, am, bm of the
+// H row index (identity off the strips), read once per tile from the LDS
+// table t[k * TOMO_MAX_NC + i] (k = 0..5: ike, ikm, ae, be, am, bm).  A stage
+// takes its row's six with v_readlane, where it would otherwise make six LDS
+// reads at a row-dependent address and wait for them.
+struct YWin {
+    float ke, ae, be, kh, ah, bh;
+    int rb;
+};
+
+__device__ __forceinline__ YWin ywin_load(const float* t, int rb, int ny, int nc) {
+    constexpr int ID = TOMO_MAX_NC - 1;
+    const int y = rb + (int)(threadIdx.x & (kWave - 1));
+    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+    YWin w;
+    w.ke = t[iye];
+    w.kh = t[TOMO_MAX_NC + iyh];
+    w.ae = t[2 * TOMO_MAX_NC + iye];
+    w.be = t[3 * TOMO_MAX_NC + iye];
+    w.ah = t[4 * TOMO_MAX_NC + iyh];
+    w.bh = t[5 * TOMO_MAX_NC + iyh];
+    w.rb = rb;
+    return w;
+}
+
+struct YCoef {
+    float ke, ae, be, kh, ah, bh;
+};
+
+__device__ __forceinline__ float lane_read(float v, int i) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+}
+
+// Coefficients of row y.  Rows below the window are the sweep's warm-up rows
+// (stage t of the first t rows), whose results never reach an output row; the
+// lane index wraps for them.
+__device__ __forceinline__ YCoef ycoef(const YWin& w, int y) {
+    const int i = (y - w.rb) & (kWave - 1);
+    return YCoef{lane_read(w.ke, i), lane_read(w.ae, i), lane_read(w.be, i),
+                 lane_read(w.kh, i), lane_read(w.ah, i), lane_read(w.bh, i)};
 }
 
 // Element access of load_cell / store_cell.  COH: relaxed atomics at agent
@@ -665,7 +750,7 @@ __device__ __forceinline__ void fill_edge(const DevSim* __restrict__ Sp, float4*
     }
 }
 
-template <int T, int G, bool BF, bool SW>
+template <int T, int G, bool BF, int WIN>
 __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcp,
@@ -710,9 +795,10 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
         cur[t] = zero_cell();
         hxp[t] = 0.f;
     }
-    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
-    auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c, SW ? &sw : nullptr); };
-    auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c, SW ? &sw : nullptr); };
+    const SpanWin sw = (WIN & kWinSpan) ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    const YWin yw = (!BF && (WIN & kWinY)) ? ywin_load(lcp + 2 * TOMO_MAX_NC, y0 - T, ny, nc) : YWin{};
+    auto load = [&](int r, Cell& c) { load_cell(S, lc4, r, x, xin, wf, c, (WIN & kWinSpan) ? &sw : nullptr); };
+    auto store = [&](int yo, const Cell& c) { store_cell(S, lc4, yo, x, wf, c, (WIN & kWinSpan) ? &sw : nullptr); };
     // FAST: every stage active (nst == T) and no source cell in reach of the tile,
     // so the stage loop carries no per-stage branch.
     auto stepg = [&](auto fast, int r, Cell in) {
@@ -724,16 +810,22 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
             if constexpr (!BF) {
                 if (FAST || t < nst) {
                     // wave-uniform row coefficients
-                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    YCoef yc;
+                    if constexpr ((WIN & kWinY) != 0) {
+                        yc = ycoef(yw, y);
+                    } else {
+                        const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                        const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                        yc = YCoef{CP(2, iye), CP(4, iye), CP(5, iye), CP(3, iyh), CP(6, iyh), CP(7, iyh)};
+                    }
                     const bool hy_ok = (y >= 0 && y < ny), ey_ok = (y >= 1 && y <= ny - 1);
                     // H half-step with the CPML recursions
                     const float dEx = lane_next(c.ez) - c.ez;
                     const float dEy = in.ez - c.ez;
                     c.phx = bxh * c.phx + axh * dEx;
-                    c.phy = CP(7, iyh) * c.phy + CP(6, iyh) * dEy;
+                    c.phy = yc.bh * c.phy + yc.ah * dEy;
                     const float ty = dEx * kxh + S.dxf * c.phx;
-                    const float tx = dEy * CP(3, iyh) + S.dyf * c.phy;
+                    const float tx = dEy * yc.kh + S.dyf * c.phy;
                     const bool hok = hx_ok && hy_ok;
                     const float hy = hok ? c.hy + S.chy * ty : 0.f;
                     const float hx = hok ? c.hx - S.chx * tx : 0.f;
@@ -742,9 +834,9 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
                     const float dHx = hx - hxp[t];
                     hxp[t] = hx;
                     c.pex = bxe * c.pex + axe * dHy;
-                    c.pey = CP(5, iye) * c.pey + CP(4, iye) * dHx;
+                    c.pey = yc.be * c.pey + yc.ae * dHx;
                     const float cy = dHy * kxe + S.dxf * c.pex;
-                    const float cx = dHx * CP(2, iye) + S.dyf * c.pey;
+                    const float cx = dHx * yc.ke + S.dyf * c.pey;
                     const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                     float en = qa.x * c.ez + qa.y * cy - qa.y * cx - qb.x * c.jp;
                     if (!FAST && wf.src && y == S.sy && x == S.sx) en += S.srcw[step0 + t];
@@ -807,8 +899,8 @@ __device__ __forceinline__ void body_edge(const TileItem& tile, const LaunchItem
     auto step_fast = [&](int r, Cell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, Cell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wf.src && S.sy >= y0 - 2 * T && S.sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, G, Cell>(y0, y1, xown, load, store, step_fast);
-    else row_stream<T, G, Cell>(y0, y1, xown, load, store, step_gen);
+    if (nst == T && !src_near) row_sweep<T, G, Cell>(y0, y1, xown, load, store, step_fast);
+    else row_sweep<T, G, Cell>(y0, y1, xown, load, store, step_gen);
 }
 
 template <int T>
@@ -829,7 +921,7 @@ __global__ __launch_bounds__(256) void k_tedge(const TileItem* __restrict__ tile
         fill_edge<T, BF>(Sp, lc4, lcp, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_edge<T, kGroupEdge, BF, true>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
+        body_edge<T, kGroupEdge, BF, kStageWin>(tile, it, nst, Sp, ordinal, lc4, lcp, ly);
     }
 }
 
@@ -1343,7 +1435,7 @@ __device__ __forceinline__ void fill_row(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T, int G, bool BF, bool SW>
+template <int T, int G, bool BF, int WIN>
 __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcy,
@@ -1378,7 +1470,8 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
     const int y0 = tile.y0, y1 = tile.y1;
     const int step0 = it.step0 + ordinal * T;
     const int k0 = ordinal * T;
-    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    const SpanWin sw = (WIN & kWinSpan) ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    const YWin yw = (!BF && (WIN & kWinY)) ? ywin_load(lcy, y0 - T, ny, nc) : YWin{};
     kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, YCell& c) {
         c.ez = c.hx = c.hy = c.jp = c.pey = c.phy = 0.f;
@@ -1390,7 +1483,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
             c.hx = fin[2 * ps + b];
             c.hy = fin[4 * ps + b];
         }
-        if constexpr (SW) {
+        if constexpr ((WIN & kWinSpan) != 0) {
             const Span sp = span_row(sw, r);
             if (x >= sp.lo && x < sp.hi) c.jp = jin[sp.off + x - sp.lo];
         } else {
@@ -1417,7 +1510,7 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
                 fout[2 * ps + b] = c.hx;
                 fout[4 * ps + b] = c.hy;
             }
-            if constexpr (SW) {
+            if constexpr ((WIN & kWinSpan) != 0) {
                 const Span sp = span_row(sw, yo);
                 if (x >= sp.lo && x < sp.hi) jout[sp.off + x - sp.lo] = c.jp;
             } else {
@@ -1453,20 +1546,27 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
             YCell c = cur[t];
             if constexpr (!BF) {
                 if (FAST || t < nst) {
-                    const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
-                    const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                    YCoef yc;
+                    if constexpr ((WIN & kWinY) != 0) {
+                        yc = ycoef(yw, y);
+                    } else {
+                        const int iye = (y >= 1 && y <= nc) ? nc - y : ((y >= ny - nc && y <= ny - 1) ? y - ny + nc : ID);
+                        const int iyh = (y >= 0 && y < nc) ? nc - 1 - y : ((y >= ny - nc && y < ny) ? y - ny + nc : ID);
+                        yc = YCoef{lcy[iye], lcy[2 * TOMO_MAX_NC + iye], lcy[3 * TOMO_MAX_NC + iye],
+                                   lcy[TOMO_MAX_NC + iyh], lcy[4 * TOMO_MAX_NC + iyh], lcy[5 * TOMO_MAX_NC + iyh]};
+                    }
                     const bool hok = (y >= 0 && y < ny), eok = (y >= 1 && y <= ny - 1);  // wave-uniform
                     const float dEx = lane_next(c.ez) - c.ez;
                     const float dEy = in.ez - c.ez;
-                    c.phy = lcy[5 * TOMO_MAX_NC + iyh] * c.phy + lcy[4 * TOMO_MAX_NC + iyh] * dEy;
-                    const float tx = dEy * lcy[TOMO_MAX_NC + iyh] + dyf * c.phy;
+                    c.phy = yc.bh * c.phy + yc.ah * dEy;
+                    const float tx = dEy * yc.kh + dyf * c.phy;
                     const float hy = hok ? c.hy + chy * dEx : 0.f;
                     const float hx = hok ? c.hx - chx * tx : 0.f;
                     const float dHy = hy - lane_prev(hy);
                     const float dHx = hx - hxp[t];
                     hxp[t] = hx;
-                    c.pey = lcy[3 * TOMO_MAX_NC + iye] * c.pey + lcy[2 * TOMO_MAX_NC + iye] * dHx;
-                    const float cx = dHx * lcy[iye] + dyf * c.pey;
+                    c.pey = yc.be * c.pey + yc.ae * dHx;
+                    const float cx = dHx * yc.ke + dyf * c.pey;
                     const float4 qa = lc4[2 * c.mat], qb = lc4[2 * c.mat + 1];
                     float en = qa.x * c.ez + qa.y * dHy - qa.y * cx - qb.x * c.jp;
                     if (!FAST && wsrc && y == sy && x == sx) en += srcw[step0 + t];
@@ -1524,8 +1624,8 @@ __device__ __forceinline__ void body_row(const TileItem& tile, const LaunchItem&
     auto step_fast = [&](int r, YCell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, YCell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, G, YCell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, G, YCell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_sweep<T, G, YCell>(y0, y1, own, load, store, step_fast);
+    else row_sweep<T, G, YCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1547,7 +1647,7 @@ __global__ __launch_bounds__(256) void k_trow(const TileItem* __restrict__ tiles
         fill_row<T, BF>(Sp, lc4, lcy, ly);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_row<T, kGroupRow, BF, true>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
+        body_row<T, kGroupRow, BF, kStageWin>(tile, it, nst, Sp, ordinal, lc4, lcy, ly);
     }
 }
 
@@ -1579,7 +1679,7 @@ __device__ __forceinline__ void fill_col(const DevSim* __restrict__ Sp, float4* 
     }
 }
 
-template <int T, int G, bool SW>
+template <int T, int G, int WIN>
 __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4, const float* __restrict__ lcx) {
@@ -1638,7 +1738,7 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
     const bool s_lane = ccol && own;
     const int ylo = max(max(y0, cy_lo), 0), yhi = min(y1, ny - D);  // rows of the contour's x legs
     const int w2 = 2 * nc;
-    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
+    const SpanWin sw = (WIN & kWinSpan) ? span_window(Sp->span, y0 - T, ny) : SpanWin{};
     kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, XCell& c) {
         c.ez = c.hx = c.hy = c.jp = c.pex = c.phx = 0.f;
@@ -1650,7 +1750,7 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
             c.hx = fin[2 * ps + b];
             c.hy = fin[4 * ps + b];
         }
-        if constexpr (SW) {
+        if constexpr ((WIN & kWinSpan) != 0) {
             const Span sp = span_row(sw, r);
             if (x >= sp.lo && x < sp.hi) c.jp = jin[sp.off + x - sp.lo];
         } else {
@@ -1675,7 +1775,7 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
                 fout[2 * ps + b] = c.hx;
                 fout[4 * ps + b] = c.hy;
             }
-            if constexpr (SW) {
+            if constexpr ((WIN & kWinSpan) != 0) {
                 const Span sp = span_row(sw, yo);
                 if (x >= sp.lo && x < sp.hi) jout[sp.off + x - sp.lo] = c.jp;
             } else {
@@ -1736,8 +1836,8 @@ __device__ __forceinline__ void body_col(const TileItem& tile, const LaunchItem&
     auto step_fast = [&](int r, XCell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, XCell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, G, XCell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, G, XCell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_sweep<T, G, XCell>(y0, y1, own, load, store, step_fast);
+    else row_sweep<T, G, XCell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1757,7 +1857,7 @@ __global__ __launch_bounds__(256) void k_tcol(const TileItem* __restrict__ tiles
         fill_col<T>(Sp, lc4, lcx);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_col<T, kGroupCol, true>(tile, it, nst, Sp, ordinal, lc4, lcx);
+        body_col<T, kGroupCol, kStageWin>(tile, it, nst, Sp, ordinal, lc4, lcx);
     }
 }
 
@@ -1780,7 +1880,7 @@ __device__ __forceinline__ void fill_inner(const DevSim* __restrict__ Sp, float4
     }
 }
 
-template <int T, int G, bool SW>
+template <int T, int G, int WIN>
 __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchItem& it, int nst,
                                          const DevSim* __restrict__ Sp, int ordinal,
                                          const float4* __restrict__ lc4) {
@@ -1809,7 +1909,7 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     const unsigned bx = (unsigned)x * 4u;  // x >= 0 on interior tiles
     typedef GAS char gbyte;
     auto at = [&](gfloat* base, unsigned off) -> gfloat& { return *(gfloat*)((gbyte*)base + off); };
-    const SpanWin sw = SW ? span_window(Sp->span, y0 - T, Sp->ny) : SpanWin{};
+    const SpanWin sw = (WIN & kWinSpan) ? span_window(Sp->span, y0 - T, Sp->ny) : SpanWin{};
     kcint* span = (kcint*)Sp->span;
     auto load = [&](int r, ICell& c) {
         gfloat* row = fin + (size_t)r * pitch;
@@ -1818,7 +1918,7 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
         c.hy = at(row + 4 * (size_t)ps, bx);
         c.mat = (mat + (size_t)r * pitch)[(unsigned)x];
         c.jp = 0.f;
-        if constexpr (SW) {
+        if constexpr ((WIN & kWinSpan) != 0) {
             const Span sp = span_row(sw, r);
             if (x >= sp.lo && x < sp.hi) c.jp = at(jin + sp.off - sp.lo, bx);
         } else {
@@ -1831,7 +1931,7 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
         at(row, bx) = c.ez;
         at(row + 2 * (size_t)ps, bx) = c.hx;
         at(row + 4 * (size_t)ps, bx) = c.hy;
-        if constexpr (SW) {
+        if constexpr ((WIN & kWinSpan) != 0) {
             const Span sp = span_row(sw, yo);
             if (x >= sp.lo && x < sp.hi) at(jout + sp.off - sp.lo, bx) = c.jp;
         } else {
@@ -1881,8 +1981,8 @@ __device__ __forceinline__ void body_inner(const TileItem& tile, const LaunchIte
     auto step_fast = [&](int r, ICell in) { return stepg(std::true_type{}, r, in); };
     auto step_gen = [&](int r, ICell in) { return stepg(std::false_type{}, r, in); };
     const bool src_near = wsrc && sy >= y0 - 2 * T && sy < y1 + T;
-    if (nst == T && !src_near) row_stream<T, G, ICell>(y0, y1, own, load, store, step_fast);
-    else row_stream<T, G, ICell>(y0, y1, own, load, store, step_gen);
+    if (nst == T && !src_near) row_sweep<T, G, ICell>(y0, y1, own, load, store, step_fast);
+    else row_sweep<T, G, ICell>(y0, y1, own, load, store, step_gen);
 
 }
 
@@ -1901,7 +2001,7 @@ __global__ __launch_bounds__(256) void k_tinner(const TileItem* __restrict__ til
         fill_inner<T>(Sp, lc4);
         __syncthreads();
         if ((int)(threadIdx.x / kWave) >= tile.nband) continue;  // after the last barrier of this tile
-        body_inner<T, kGroupInner, true>(tile, it, nst, Sp, ordinal, lc4);
+        body_inner<T, kGroupInner, kStageWin>(tile, it, nst, Sp, ordinal, lc4);
     }
 }
 
@@ -1940,10 +2040,10 @@ __global__ __launch_bounds__(256, kFusedWgPerCu<T>) void k_tfused(const TileItem
         // Boundary tiles set the launch's length (their stages carry the CPML
         // and contour work): their waves issue first on a shared SIMD.
         if (cls != TILE_INNER) __builtin_amdgcn_s_setprio(3);
-        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF, false>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
-        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF, false>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
-        else if (cls == TILE_COL) body_col<T, kFusedGroupCol, false>(tile, it, nst, Sp, ordinal, lc4, ltab);
-        else body_inner<T, kFusedGroupInner, false>(tile, it, nst, Sp, ordinal, lc4);
+        if (cls == TILE_CORNER) body_edge<T, kFusedGroupEdge, BF, kFusedWin>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_ROW) body_row<T, kFusedGroupRow, BF, kFusedWin>(tile, it, nst, Sp, ordinal, lc4, ltab, ly);
+        else if (cls == TILE_COL) body_col<T, kFusedGroupCol, kFusedWin>(tile, it, nst, Sp, ordinal, lc4, ltab);
+        else body_inner<T, kFusedGroupInner, kFusedWin>(tile, it, nst, Sp, ordinal, lc4);
         __builtin_amdgcn_s_setprio(0);
     }
 }
