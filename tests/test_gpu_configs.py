@@ -120,7 +120,7 @@ def test_cfg1_full_size_vs_fp64_oracle(oracle):
 
 def test_cfg2_full_size_plain_schedule_vs_fp64_oracle(oracle):
     """1024^2, dx 0.25 mm, 8192 steps under the bench's plain schedule (fused
-    T = 4, 1-row boundary, 4-row column-edge and 14-row interior tiles): fields
+    T = 4, 1-row boundary, 4-row column-edge and 10-row interior tiles): fields
     at step 2048 and at the end, D and fitness against the FP64 oracle."""
     cfg = t.preset(t.PRESET_CFG2)
     v = seeded_vectors(1, 8, 20261202)[0]

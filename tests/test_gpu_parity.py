@@ -91,7 +91,7 @@ def test_phys_fields_vs_oracle(ctx, oracle):
                                    (4, 16, 1, 32, 3, 8, t.FORM_FUSED), (8, 8, 4, 64, 4, 16, t.FORM_FUSED, 2),
                                    (4, 32, 1, 64, 4, 16, t.FORM_STREAMS, 3), (6, 48, 1, 60, 4, 16, t.FORM_STREAMS, 24),
                                    (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 8), (2, 5, 2, 32, 4, 4, t.FORM_STREAMS, 2, 12),
-                                   (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 14),
+                                   (4, 4, 1, 64, 1, 16, t.FORM_FUSED, 1, 14), (4, 4, 1, 60, 1, 16, t.FORM_FUSED, 1, 10),
                                    (4, 8, 1, 32, 1, 16, t.FORM_RESIDENT), (8, 12, 1, 64, 1, 16, t.FORM_RESIDENT),
                                    (2, 12, 1, 32, 1, 16, t.FORM_RESIDENT), (4, 12, 1, 64, 1, 16, t.FORM_RESIDENT),
                                    (4, 4, 1, 64, 1, 16, t.FORM_TILE), (2, 5, 1, 32, 1, 16, t.FORM_TILE),

@@ -39,10 +39,12 @@ constexpr int kGroupInner = 4, kGroupRow = 1, kGroupCol = 1, kGroupEdge = 1, kGr
 // two interior rows per group measured 0.8% faster than four on 1024^2 with
 // 14-row interior tiles).
 constexpr int kFusedGroupInner = 2, kFusedGroupRow = 1, kFusedGroupCol = 1, kFusedGroupEdge = 1;
-// Row spans: the per-class kernels read them from a per-lane window (SpanWin,
-// 3% faster on a 1024 x 512^2 batch); the fused kernel keeps scalar loads,
-// since the window's registers made its bodies spill more (1024^2: 21.6
-// against 21.1 us per launch).
+// Row windows: the per-class kernels read the rows' dispersive spans from a
+// per-lane window (SpanWin, 3% faster on a 1024 x 512^2 batch) and so do the
+// corner and row-edge stages their y-strip CPML coefficients (YWin; plain
+// 1024^2 186 -> 198 Gcell-steps/s, the batch 485 -> 507).  The fused kernel
+// keeps scalar span loads: with its 10-row interior tiles the span window
+// measured the same (211.8 against 212.6 Gcell-steps/s).
 // Window flags of the row-sweep bodies: kWinSpan reads the rows' dispersive
 // spans from a SpanWin, kWinY the y-strip CPML coefficients from a YWin.
 constexpr int kWinSpan = 1, kWinY = 2;

@@ -1699,7 +1699,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
             if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
     }
     const int Ts[] = {1, 2, 4, 6, 8}, Ss[] = {4, 6, 8, 12, 16, 32, 48, 64, 128}, Ws[] = {1, 2, 4}, Gs[] = {2, 4, 8, 16};
-    const int ERs[] = {1, 2, 3, 4, 8, 16, 24}, SIs[] = {0, 8, 12, 14, 16, 24};
+    const int ERs[] = {1, 2, 3, 4, 8, 16, 24}, SIs[] = {0, 8, 10, 12, 14, 16, 24};
     const int Fs[] = {TOMO_FORM_STREAMS, TOMO_FORM_FUSED};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         if (tc.mode == TOMO_MODE_PHYSICAL)
