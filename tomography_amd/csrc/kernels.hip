@@ -36,9 +36,9 @@ constexpr int kWave = 64;
 // batches depends on occupancy more than on prefetch depth.
 constexpr int kGroupInner = 4, kGroupRow = 1, kGroupCol = 1, kGroupEdge = 1, kGroupRef = 2;
 // Prefetch groups of the fused kernel (one or a few simulations, latency-bound;
-// two interior rows per group measured 0.8% faster than four on 1024^2 with
-// 14-row interior tiles).
-constexpr int kFusedGroupInner = 2, kFusedGroupRow = 1, kFusedGroupCol = 1, kFusedGroupEdge = 1;
+// on 1024^2 with 10-row interior tiles one interior row per group measured
+// 1.1% faster than two, and two 0.8% faster than four with 14-row tiles).
+constexpr int kFusedGroupInner = 1, kFusedGroupRow = 1, kFusedGroupCol = 1, kFusedGroupEdge = 1;
 // Row windows: the per-class kernels read the rows' dispersive spans from a
 // per-lane window (SpanWin, 3% faster on a 1024 x 512^2 batch) and so do the
 // corner and row-edge stages their y-strip CPML coefficients (YWin; plain
