@@ -6,6 +6,10 @@
         per-kernel, per-template average of COUNTER (KiB per dispatch) from
         PREFIX_counter_collection.csv, plus the copy-kernel dispatches used
         for calibration
+    python -m tomography_amd.rocprof_summary traffic FETCH.json WRITE.json BENCH.json COPY_MB COMMAND...
+        profiles/update_traffic.json from the two `pmc` summaries (FETCH_SIZE,
+        WRITE_SIZE) and the profiled run's bench line, scaled by the copy of
+        COPY_MB MiB the run made (bench.py --calib-copy-mb)
     python -m tomography_amd.rocprof_summary db FILE.db [KERNEL=LAST ...]
         the same per-kernel figures as `trace` from rocprofv3's default
         SQLite output (PREFIX_results.db)
@@ -124,7 +128,46 @@ def db(path, spec=()):
     return out
 
 
+def traffic(fetch_path, write_path, bench_path, copy_mb, command):
+    """HBM bytes per launch of the update kernels (profiles/update_traffic.json)
+    from a FETCH_SIZE and a WRITE_SIZE summary (`pmc` output, each with the
+    timed region's dispatches) and the bench line of the profiled command.
+    Each counter is scaled by what it reads for the largest copy dispatch of
+    the run (bench.py --calib-copy-mb copy_mb: a copy of copy_mb MiB)."""
+    fetch, write = json.load(open(fetch_path)), json.load(open(write_path))
+    lines = [l for l in open(bench_path) if l.lstrip().startswith("{")]
+    bench = json.loads(lines[-1])
+    copy_bytes = int(copy_mb) * 1024 * 1024
+    fcopy, wcopy = fetch["largest_copy_dispatches"][0]["kib"], write["largest_copy_dispatches"][0]["kib"]
+    ff, wf = copy_bytes / 1024.0 / fcopy, copy_bytes / 1024.0 / wcopy
+    out = {"calibration": {"copy_bytes": copy_bytes, "fetch_kib": fcopy, "write_kib": wcopy,
+                           "fetch_factor": ff, "write_factor": wf},
+           "command": command}
+    sched = lambda s: [s[k] for k in ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_mult",
+                                      "form", "edge_rows", "inner_rows")]
+    legs = [("plain_cfg2", bench["config"]["schedule"], bench["roofline"])]
+    if bench.get("batched"):
+        legs.append(("batched_cfg3", bench["batched"]["schedule"], bench["batched"]["roofline"]))
+    for name, s, rl in legs:
+        T = s["tsteps"]
+        kern = {}
+        for r in [rl] + list(rl.get("other_kernels", [])):
+            key = f"{r['kernel']}<{T}>"
+            fk = fetch["kernels"].get(key, {}).get("avg_kib_last")
+            wk = write["kernels"].get(key, {}).get("avg_kib_last")
+            if fk is None or wk is None:
+                continue
+            kern[r["kernel"]] = {"fetch_kib": fk, "write_kib": wk,
+                                 "hbm_bytes_per_launch": (fk * ff + wk * wf) * 1024.0,
+                                 "alg_bytes_per_launch": r["alg_bytes_per_launch"]}
+        out[name] = {"schedule": sched(s), "kernels": kern}
+    return out
+
+
 def main(argv):
+    if len(argv) >= 6 and argv[0] == "traffic":
+        print(json.dumps(traffic(argv[1], argv[2], argv[3], argv[4], " ".join(argv[5:])), indent=1))
+        return 0
     if len(argv) >= 3 and argv[0] == "stats":
         print(json.dumps(stats(argv[1], argv[2]), indent=1))
     elif len(argv) >= 3 and argv[0] == "trace":
