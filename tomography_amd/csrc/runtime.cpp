@@ -1692,7 +1692,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     // poor corner of the space: one simulation (fused, short tiles) and many
     // (four streams, tall tiles; T = 6 where the chunk allows it).
     if (tc.mode == TOMO_MODE_PHYSICAL) {
-        const tomo_schedule seeds[3] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 14},
+        const tomo_schedule seeds[3] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 10},
                                         {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0},
                                         {6, 48, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0}};
         for (const tomo_schedule& sd : seeds)
