@@ -527,8 +527,13 @@ void build_tiles(Slot* sl, int item, int T, int S, int Si, int W, int Wi, int ER
     // clean rows / columns lie strictly between the special ones
     const int ybot = std::max(nc, D + 1) + 1, ytop = std::min(ny - nc, ny - D - 1) - 1;
     const int xbot = std::max(nc, D + 1) + 1, xtop = std::min(nx - nc, nx - D - 1) - 1;
-    const int ixl = xbot + T;                         // first interior owned column
-    const int nbi = std::max(0, (xtop - T + 1 - ixl) / V);  // full interior bands
+    const int ixl0 = xbot + T;                        // first column whose T-step cone is clean
+    const int ixmax = xtop - T + 1;                   // end of such columns
+    const int nbi = std::max(0, (ixmax - ixl0) / V);  // full interior bands
+    // The full bands sit where both column-edge regions (left of them and
+    // right of them) fit in one band each if they can: a column-edge lane
+    // costs about three interior ones, and a third band would be mostly idle.
+    const int ixl = nbi > 0 ? std::min(std::max(ixl0, nx + 1 - nbi * V - V), ixmax - nbi * V) : ixl0;
     const int ixr = ixl + nbi * V;                    // end of interior owned columns
     const int ia = ybot + T, ib = ytop - T + 1;       // interior output rows [ia, ib)
     const bool split = d.mode == TOMO_MODE_PHYSICAL && nbi > 0 && ib > ia;
