@@ -86,6 +86,36 @@ def test_phys_fields_vs_oracle(ctx, oracle):
         assert rel_l2(a, b64) < FIELD_TOL_F64, (name, rel_l2(a, b64))
 
 
+@pytest.mark.parametrize("n, steps, sched", [(256, 300, (4, 4, 1, 60, 1, 16, t.FORM_FUSED, 1, 10)),
+                                             (512, 700, (6, 48, 1, 60, 4, 16, t.FORM_STREAMS, 24)),
+                                             (512, 700, (8, 12, 1, 64, 2, 16, t.FORM_STREAMS, 4, 40))])
+def test_placed_interior_bands_vs_oracle(oracle, n, steps, sched):
+    """The full interior bands need not start at the first clean column: the
+    tile builder moves them right so that the column-edge regions on both
+    sides fit in one band each (256^2 at T = 4: 33 and 56 columns instead of
+    21 and 68; 512^2 at T = 6: 45 and 52 instead of 23 and 74; at T = 8: 33
+    and 48 instead of 25 and 56).  The fields must still match the oracle and
+    equal those of the streams form at T = 4 with 32-row tiles (whose bands
+    stay in place on 512^2: 21 and 44 columns).  By the last step the pulse
+    has reached both column-edge regions (checked on the oracle's Ez)."""
+    cfg = phys_small(steps=steps, n=n, n_exc=1)
+    with t.Context() as ctx:
+        ctx.set_schedule(*sched)
+        g = ctx.sim(cfg, PHYS_PARAMS, exc=0)
+        g.step(steps)
+        got = g.fields()
+    with t.Context() as base:
+        base.set_schedule(4, 32, 4, 32, form=t.FORM_STREAMS)
+        b = base.sim(cfg, PHYS_PARAMS, exc=0)
+        b.step(steps)
+        ref = b.fields()
+    o32 = oracle_fields(oracle, cfg, PHYS_PARAMS, 0, steps, False).fields()
+    assert np.abs(o32[0][:, :60]).max() > 1e-7 and np.abs(o32[0][:, -60:]).max() > 1e-4
+    for name, a, r, o in zip(("ez", "hx", "hy", "jp"), got, ref, o32):
+        assert np.array_equal(a, r), name
+        assert rel_l2(a, o) < FIELD_TOL_F32, (name, rel_l2(a, o))
+
+
 @pytest.mark.parametrize("sched", [(1, 32, 4, 32), (2, 16, 4, 32), (4, 7, 1, 32), (6, 50, 3, 48), (8, 64, 2, 64),
                                    (2, 8, 2, 32, 4, 2, t.FORM_FUSED), (8, 5, 4, 64, 1, 1, t.FORM_FUSED),
                                    (4, 16, 1, 32, 3, 8, t.FORM_FUSED), (8, 8, 4, 64, 4, 16, t.FORM_FUSED, 2),
