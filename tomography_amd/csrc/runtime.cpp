@@ -1699,7 +1699,7 @@ int tomo_autotune(tomo_ctx* c, const tomo_sim_config* cfg, int n_sims, tomo_sche
     if (tc.mode == TOMO_MODE_PHYSICAL) {
         const tomo_schedule seeds[3] = {{4, 4, 1, 1, c->K, 16, TOMO_FORM_FUSED, 1, 10},
                                         {4, 32, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0},
-                                        {6, 48, 1, 4, c->K, 16, TOMO_FORM_STREAMS, 24, 0}};
+                                        {6, 52, 1, 4, c->K, 32, TOMO_FORM_STREAMS, 24, 0}};
         for (const tomo_schedule& sd : seeds)
             if (!rc && c->K % sd.tsteps == 0) rc = trial(sd);
     }

@@ -37,14 +37,14 @@ SCHED_KEYS = ("tsteps", "seg_rows", "edge_waves", "inner_waves", "chunk", "grid_
 # one wave per workgroup, 1-row corner / row-edge tiles, 4-row column-edge
 # tiles and 10-row interior tiles (on 1024^2 the most tiles that still run in
 # one round of three waves per SIMD).  The batched one: four streams, T = 6
-# blocks, 48-row column-edge and interior tiles of four waves, 24-row
-# row-edge tiles.  Both fold the running DFT over 60-step chunks: the chunk
+# blocks, 52-row column-edge and interior tiles (the 64-row cap) of four
+# waves, 24-row row-edge tiles, 32 persistent workgroups per CU.  Both fold the running DFT over 60-step chunks: the chunk
 # sets the FP32 partial sums, so equal chunks make batched and plain results
 # equal bit for bit.  The drop-in one: resident T = 8 tiles
 # of 12 rows over 64-step chunks.
 PLAIN_SCHEDULE = dict(tsteps=4, seg_rows=4, edge_waves=1, inner_waves=1, chunk=60, grid_mult=16, form=2, edge_rows=1,
                       inner_rows=10)
-BATCH_SCHEDULE = dict(tsteps=6, seg_rows=48, edge_waves=1, inner_waves=4, chunk=60, grid_mult=16, form=1,
+BATCH_SCHEDULE = dict(tsteps=6, seg_rows=52, edge_waves=1, inner_waves=4, chunk=60, grid_mult=32, form=1,
                       edge_rows=24, inner_rows=0)
 REF_SCHEDULE = dict(tsteps=8, seg_rows=12, edge_waves=1, inner_waves=1, chunk=64, grid_mult=16, form=3, edge_rows=1,
                     inner_rows=0)
