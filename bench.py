@@ -557,6 +557,8 @@ def main():
     ap.add_argument("--plain-evals", type=int, default=2,
                     help="plain-path evals on the batched workload (for the batched/plain ratio; 0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
+    ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
+                    help="device-memory budget of the context (GB; 0: the library's default of 90%% of free memory)")
     ap.add_argument("--uneven", type=int, default=128,
                     help="requests of the uneven-batch study (512 in its definition, bounded by default; 0 = skip)")
     ap.add_argument("--uneven-plain-sample", type=int, default=2,
@@ -571,7 +573,7 @@ def main():
               "(simulations are independent: more GPUs would each run their own share).", file=sys.stderr)
         return 2
     ref_full = None if (args.no_ref or args.only_ref) else start_reference_full_run()
-    ctx = t.Context()
+    ctx = t.Context(hbm_budget=int(args.hbm_budget_gb * 1e9))
     rsched = dict(REF_SCHEDULE)
     if args.ref_schedule:
         rsched = dict(zip(SCHED_KEYS, [int(x) for x in args.ref_schedule.split(",")]))
