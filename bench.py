@@ -569,7 +569,7 @@ def main():
                     help="run only the reference-compatible (drop-in) leg and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: tomography_amd runs on one GPU; --gpus {args.gpus} is not supported "
               "(simulations are independent: more GPUs would each run their own share).", file=sys.stderr)
         return 2
     ref_full = None if (args.no_ref or args.only_ref) else start_reference_full_run()

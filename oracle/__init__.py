@@ -1,5 +1,5 @@
 """Correctness oracle for the 2D TMz FDTD fitness path (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
-package, and only as the checker or the timed CPU baseline.
+package, and only to check results or as the timed CPU baseline.
 """

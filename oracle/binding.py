@@ -1,6 +1,7 @@
 """ctypes binding of oracle/liboracle.so and oracle/_ref/libfdtd_ref.so.
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP product, never the product.
+TEST INFRASTRUCTURE ONLY: what the HIP product's results are checked against,
+never the product.
 """
 import ctypes
 import os

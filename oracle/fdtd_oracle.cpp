@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
-// as the checker (or the timed CPU baseline), never as the product.
+// to check results (or as the timed CPU baseline), never as the product.
 //
 // Two kinds of arithmetic, chosen by the template parameter R:
 //   R = float  : restates the reference's own FP32 storage with the FP64

@@ -3,7 +3,8 @@
 Runs the reference's own functions (FDTD_CPU.cxx, FDTD_common.cxx compiled
 unmodified into oracle/_ref by oracle/Makefile) on seeded inputs and stores
 small input/output vectors, so that tests on machines without the reference
-tree (the GPU box) can still check against the reference:
+tree (a GPU machine with only this repository) can still check against the
+reference:
 
   ref_cpu_cpml.npz    CPML tables kex, kmx, aex, bex, amx, bmx (FDTD_CPU.cxx:765-776)
   ref_cpu_run.npz     162-vector; material patch map; per-material coefficient

@@ -32,7 +32,7 @@ pytestmark = pytest.mark.gpu
 
 FF_TOL = 1e-4
 FIELD_TOL_F64 = 1e-4
-# the box gives one GPU 16 host cores
+# at most 16 host worker threads beside the one GPU
 POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
 # a candidate far from every preset's seeded target (well-conditioned fitness)
 FAR = np.array([-8.0, 9.0, 5.0, 28.0, 45.0, 15.0, 0.5, 1.0])
